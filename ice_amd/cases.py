@@ -1,0 +1,359 @@
+"""Synthetic cases: blocks, boundary tables, states and gas fields in the
+layouts of include/ice_gpu.h, built the way ICE's set-up builds them.
+
+The boundary table follows Setup_BC (src/lib/io/IO_BC.f90:65-344): block by
+block, every face cell of the block (2(nj nk + ni nk + ni nj) of them), faces 1..6,
+one copy for all families, record idx = (n-1)*ngroups + p.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import geometry
+from .native import (AG, GC, HLLE, IG, MK, NCOND, RD_ALPHA, RD_BETA, RD_MASSFLUX, RD_RADIUS, RD_TEMPERATURE,
+                     RD_VELOCITY, REC_ND, REC_NI, RUSANOV, SAUREL, Config, Family, R_B, R_BS, R_DONOR0, R_F, R_FS,
+                     R_I, R_IS, R_J, R_JS, R_K, R_KS, R_NCHI1, R_NCHI2, R_NORMAL_DIR, R_P, R_TYPE)
+
+PI = 3.14159265358979323846
+
+
+@dataclasses.dataclass
+class Block:
+    dims: tuple
+    geo: dict
+    prim: List[np.ndarray]            # per family, flat Fortran (nc, -1:ni+2, -1:nj+2, -1:nk+2)
+    gas: Optional[dict] = None        # prim (5,ni,nj,nk), R, gam, k, mu flat Fortran
+    yinf: Optional[np.ndarray] = None
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    cfg: Config
+    fams: List[Family]
+    blocks: List[Block]
+    reci: np.ndarray                  # (nrec, REC_NI) int32
+    recd: np.ndarray                  # (nrec, REC_ND) float64
+    donor: np.ndarray                 # (nd, 4) int32
+    weight: np.ndarray                # (nd,)
+    steps: int = 1
+    meta: Dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def cells(self) -> int:
+        return sum(b.dims[0] * b.dims[1] * b.dims[2] for b in self.blocks)
+
+    def cell_stages(self, steps: int) -> int:
+        """interior cells x families x RK stages x steps (the Mcell-stage count)"""
+        return self.cells * self.cfg.ngroups * self.cfg.nrk * steps
+
+
+def make_config(**kw) -> Config:
+    c = Config()
+    c.ngroups = kw.get("ngroups", 1)
+    c.nrk = kw.get("nrk", 3)
+    c.time_accurate = int(kw.get("time_accurate", True))
+    c.cfl_ramp_iter = kw.get("cfl_ramp_iter", 0)
+    c.ndir = kw.get("ndir", 3)
+    c.limiter = kw.get("limiter", 3)
+    c.shock_detector = int(kw.get("shock_detector", False))
+    c.irs = int(kw.get("irs", False))
+    c.coupled = int(kw.get("coupled", False))
+    c.drag_select = kw.get("drag_select", 13)
+    c.heat_select = kw.get("heat_select", 8)
+    c.blow_select = kw.get("blow_select", 0)
+    c.mode = kw.get("mode", 0)
+    c.var_yinf = int(kw.get("var_yinf", False))
+    c.cfl = kw.get("cfl", 0.5)
+    c.dt_max = kw.get("dt_max", 1e-4)
+    c.tau_factor = kw.get("tau_factor", 1.0)
+    c.irs_beta = kw.get("irs_beta", 0.5)
+    return c
+
+
+def make_family(closure: int, riemann: Optional[int] = None, rho_mat=2700.0, cs=1598.0, lv=10.8e6, emiss=0.0,
+                frozen=(), evap=0, intf=0, ep=None) -> Family:
+    f = Family()
+    f.closure = closure
+    f.riemann = (SAUREL if closure == MK else RUSANOV) if riemann is None else riemann
+    f.nfrozen = len(frozen)
+    for m, s in enumerate(frozen):
+        f.frozen[m] = s
+    f.evap_select, f.intf_select, f.surrogate = evap, intf, 0
+    f.rho_mat, f.cs, f.lv, f.emiss = rho_mat, cs, lv, emiss
+    for m, v in enumerate(ep if ep is not None else [0.0] * 8):
+        f.ep[m] = v
+    return f
+
+
+def face_cells(dims):
+    """(f, i, j, k) of every face cell of a block, faces 1..6, first tangential index fastest."""
+    ni, nj, nk = dims
+    out = []
+    for f in range(1, 7):
+        if f <= 2:
+            i = 1 if f == 1 else ni
+            out += [(f, i, j, k) for k in range(1, nk + 1) for j in range(1, nj + 1)]
+        elif f <= 4:
+            j = 1 if f == 3 else nj
+            out += [(f, i, j, k) for k in range(1, nk + 1) for i in range(1, ni + 1)]
+        else:
+            k = 1 if f == 5 else nk
+            out += [(f, i, j, k) for j in range(1, nj + 1) for i in range(1, ni + 1)]
+    return out
+
+
+class TableBuilder:
+    """Accumulates the boundary table of a case, block by block."""
+
+    def __init__(self, ngroups: int):
+        self.ng = ngroups
+        self.cells = []     # per face cell: (b, i, j, k, f, type, extra ints, extra reals)
+        self.donor = []
+        self.weight = []
+
+    def add(self, b, f, i, j, k, code, ints=None, reals=None):
+        self.cells.append((b, i, j, k, f, code, dict(ints or {}), dict(reals or {})))
+
+    def arrays(self):
+        n = len(self.cells) * self.ng
+        reci = np.zeros((n, REC_NI), np.int32)
+        recd = np.zeros((n, REC_ND), np.float64)
+        for c, (b, i, j, k, f, code, ints, reals) in enumerate(self.cells):
+            for p in range(1, self.ng + 1):
+                r = c * self.ng + (p - 1)
+                reci[r, R_B], reci[r, R_I], reci[r, R_J], reci[r, R_K] = b, i, j, k
+                reci[r, R_F], reci[r, R_TYPE], reci[r, R_P] = f, code, p
+                for key, v in ints.items():
+                    reci[r, key] = v
+                for key, v in reals.items():
+                    recd[r, key] = v
+        donor = np.asarray(self.donor, np.int32).reshape(-1, 4) if self.donor else np.zeros((0, 4), np.int32)
+        weight = np.asarray(self.weight, np.float64)
+        return reci, recd, donor, weight
+
+
+def padded_state(dims, nc, fill_fn) -> np.ndarray:
+    """Flat Fortran (nc, -1:ni+2, -1:nj+2, -1:nk+2) array, NaN ghosts, interior from fill_fn(i,j,k arrays) -> (nc, ...)"""
+    ni, nj, nk = dims
+    a = np.full((nc, ni + 4, nj + 4, nk + 4), np.nan)
+    I, J, K = np.meshgrid(np.arange(1, ni + 1), np.arange(1, nj + 1), np.arange(1, nk + 1), indexing="ij")
+    a[:, 2:ni + 2, 2:nj + 2, 2:nk + 2] = fill_fn(I, J, K)
+    return np.ascontiguousarray(a.ravel(order="F"))
+
+
+def gas_arrays(prim5: np.ndarray, R, gam, k, mu) -> dict:
+    """prim5: (5, ni, nj, nk) rho u v w T; scalars or (ni,nj,nk) arrays for the rest."""
+    shp = prim5.shape[1:]
+    full = lambda x: np.broadcast_to(np.asarray(x, float), shp)  # noqa: E731
+    f = lambda a: np.ascontiguousarray(np.asarray(a, float).ravel(order="F"))  # noqa: E731
+    return {"prim": f(prim5), "R": f(full(R)), "gam": f(full(gam)), "k": f(full(k)), "mu": f(full(mu))}
+
+
+# ---------------------------------------------------------------- case 1
+def case_riemann_1d(n: int = 2000, steps: int = 1000, limiter: int = 1, drag: bool = True, mode: int = 0) -> Case:
+    """Plain method, CPU-runnable: 1-D block of n cells on [0,1], IG, Riemann problem
+    split at x = 0.5, frozen uniform gas at rest, MUSCL minmod, Rusanov, SSP-RK3,
+    CFL 0.5, time-accurate, transmissive (400) ends, type-0 side faces.
+
+    Granular energy e maps to IG's pseudo-pressure as P = rho e / 1.5
+    (e = 3 theta / 2 with P = rho theta, cons(5) = rho |u|^2 + 3 P)."""
+    dims = (n, 1, 1)
+    dx = 1.0 / n
+    nodes = geometry.cartesian_nodes(np.linspace(0.0, 1.0, n + 1), [0.0, dx], [0.0, dx])
+    geo = geometry.metrics(nodes)
+    T = 300.0
+
+    def fill(I, J, K):
+        x = (I - 0.5) * dx
+        left = x < 0.5
+        rho = np.where(left, 1e-3, 5e-4)
+        num = np.where(left, 1e9, 5e8)
+        u = np.where(left, 10.0, -10.0)
+        e = np.where(left, 1.0, 0.5)
+        z = np.zeros_like(x)
+        return np.stack([rho, u, z, z, rho * e / 1.5, z + T, num])
+
+    prim = padded_state(dims, 7, fill)
+    g5 = np.zeros((5,) + dims)
+    g5[0] = 1.2
+    g5[4] = 300.0
+    gas = gas_arrays(g5, 287.05, 1.4, 0.026, 1.8e-5) if drag else None
+    tb = TableBuilder(1)
+    for (f, i, j, k) in face_cells(dims):
+        tb.add(1, f, i, j, k, 400 if f <= 2 else 0)
+    reci, recd, donor, weight = tb.arrays()
+    cfg = make_config(ngroups=1, nrk=3, time_accurate=True, ndir=1, limiter=limiter, coupled=drag,
+                      drag_select=4, heat_select=8, cfl=0.5, mode=mode)
+    fam = make_family(IG, RUSANOV)
+    return Case("riemann1d", cfg, [fam], [Block(dims, geo, [prim], gas)], reci, recd, donor, weight, steps,
+                meta={"dx": dx})
+
+
+# ---------------------------------------------------------------- case 2
+def fourier_velocity(X, Y, seed: int, modes: int = 16, umax: float = 5.0):
+    """Divergence-free-looking 2-D field: sum of `modes` sine modes with wave vectors
+    2 pi (kx, ky), kx, ky in 1..8, random amplitudes and phases, scaled to max |u| = umax."""
+    rng = np.random.default_rng(seed)
+    u = np.zeros_like(X)
+    v = np.zeros_like(X)
+    for _ in range(modes):
+        kx, ky = rng.integers(1, 9, size=2)
+        amp = rng.normal()
+        ph = rng.uniform(0, 2 * np.pi)
+        th = rng.uniform(0, 2 * np.pi)
+        s = np.sin(2 * np.pi * (kx * X + ky * Y) + ph)
+        u += amp * np.cos(th) * s
+        v += amp * np.sin(th) * s
+    mag = np.sqrt(u * u + v * v).max()
+    return u * umax / mag, v * umax / mag
+
+
+def case_crossing_jets(n: int = 1024, steps: int = 500, seed: int = 20261204, mode: int = 0,
+                       time_accurate: bool = True) -> Case:
+    """2-D crossing jets: n x n cells on the unit square, AG, HLLE, MUSCL van Leer
+    with the Jameson sensor, SSP-RK3, FP64.  Two 402 inlet jets on face 1 at +-30
+    degrees (y in [0.25,0.35] and [0.65,0.75]), rho_p = 1e-2, n = 1e10, |u| = 20;
+    the rest of face 1 symmetry (300), faces 2-4 extrapolation (400), faces 5/6
+    planar (0).  Frozen gas: 16 random Fourier modes (max |u| = 5 m/s), 300 K.
+    Drag Schiller-Naumann, heat Ranz-Marshall.  The domain starts empty."""
+    dims = (n, n, 1)
+    d = 1.0 / n
+    nodes = geometry.cartesian_nodes(np.linspace(0, 1, n + 1), np.linspace(0, 1, n + 1), [0.0, d])
+    geo = geometry.metrics(nodes)
+    rho_mat = 1000.0
+    rho_j, n_j, spd = 1e-2, 1e10, 20.0
+    radius = (0.75 * rho_j / (n_j * PI * rho_mat)) ** (1.0 / 3.0)
+    xc = (np.arange(1, n + 1) - 0.5) * d
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    ug, vg = fourier_velocity(X, Y, seed)
+    g5 = np.zeros((5,) + dims)
+    g5[0] = 1.2
+    g5[1, :, :, 0] = ug
+    g5[2, :, :, 0] = vg
+    g5[4] = 300.0
+    gas = gas_arrays(g5, 287.05, 1.4, 0.026, 1.8e-5)
+    rho0 = 1e-8
+    m_drop = rho_mat * 4.0 / 3.0 * PI * radius ** 3
+
+    def fill(I, J, K):
+        z = np.zeros(I.shape)
+        uu = ug[I - 1, J - 1]
+        vv = vg[I - 1, J - 1]
+        th = rho0 * 1e-2
+        return np.stack([z + rho0, uu, vv, z, z + th, z, z, z + th, z, z + th, z + 300.0, z + rho0 / m_drop])
+
+    prim = padded_state(dims, 12, fill)
+    tb = TableBuilder(1)
+    g = rho_j * spd * math.cos(math.radians(30.0))
+    for (f, i, j, k) in face_cells(dims):
+        if f == 1:
+            y = (j - 0.5) * d
+            if 0.25 <= y <= 0.35 or 0.65 <= y <= 0.75:
+                ang = math.radians(30.0) if y < 0.5 else -math.radians(30.0)
+                tb.add(1, f, i, j, k, 402, reals={RD_MASSFLUX: g, RD_VELOCITY: spd, RD_ALPHA: ang, RD_BETA: 0.0,
+                                                  RD_TEMPERATURE: 300.0, RD_RADIUS: radius})
+            else:
+                tb.add(1, f, i, j, k, 300)
+        elif f <= 4:
+            tb.add(1, f, i, j, k, 400)
+        else:
+            tb.add(1, f, i, j, k, 0)
+    reci, recd, donor, weight = tb.arrays()
+    cfg = make_config(ngroups=1, nrk=3, time_accurate=time_accurate, ndir=2, limiter=3, shock_detector=True,
+                      coupled=True, drag_select=4, heat_select=6, cfl=0.5, mode=mode)
+    fam = make_family(AG, HLLE, rho_mat=rho_mat, cs=4186.0, lv=2.26e6)
+    return Case("crossing_jets", cfg, [fam], [Block(dims, geo, [prim], gas)], reci, recd, donor, weight, steps,
+                meta={"radius": radius})
+
+
+# ---------------------------------------------------------------- driving
+class Runner:
+    """Feeds a Case to a library with the ice_gpu_ C-ABI (prefix 'ice_gpu_') or its
+    CPU restatement (prefix 'orc_'), and reads states back."""
+
+    def __init__(self, lib, case: Case, prefix: str = "ice_gpu_", device: int = 0):
+        import ctypes as C
+        self.lib, self.case, self.pre = lib, case, prefix
+        self.C = C
+        fams = (Family * len(case.fams))(*case.fams)
+        dims = np.asarray([d for b in case.blocks for d in b.dims], np.int32)
+        ctx = C.c_void_p()
+        rc = self.f("create")(device, C.byref(case.cfg), fams, len(case.blocks), dims, C.byref(ctx))
+        self.ctx = ctx
+        self.check(rc, "create")
+        for b, B in enumerate(case.blocks, start=1):
+            g = B.geo
+            self.check(self.f("set_geometry")(ctx, b, g["vol"], g["dl"], g["M"], g["face_i"], g["face_j"],
+                                              g["face_k"]), "set_geometry")
+        self.check(self.f("set_boundaries")(ctx, case.reci.shape[0], np.ascontiguousarray(case.reci),
+                                            np.ascontiguousarray(case.recd), case.donor.shape[0],
+                                            np.ascontiguousarray(case.donor.reshape(-1)),
+                                            np.ascontiguousarray(case.weight)), "set_boundaries")
+        for b, B in enumerate(case.blocks, start=1):
+            for p in range(1, case.cfg.ngroups + 1):
+                self.check(self.f("put_state")(ctx, b, p, B.prim[p - 1]), "put_state")
+            if B.gas is not None and case.cfg.coupled:
+                gg = B.gas
+                self.check(self.f("put_gas")(ctx, b, gg["prim"], gg["R"], gg["gam"], gg["k"], gg["mu"]), "put_gas")
+            if B.yinf is not None and case.cfg.var_yinf:
+                self.check(self.f("put_gas_yinf")(ctx, b, B.yinf), "put_gas_yinf")
+
+    def f(self, name):
+        return getattr(self.lib, self.pre + name)
+
+    def error(self) -> str:
+        buf = self.C.create_string_buffer(512)
+        self.f("last_error")(self.ctx, buf, 512)
+        return buf.value.decode()
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{self.pre}{what} failed ({rc}): {self.error()}")
+
+    def step(self, n: int):
+        from .native import Status
+        st = Status()
+        rc = self.f("step")(self.ctx, n, self.C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"{self.pre}step failed ({rc}): {self.error()}")
+        return st
+
+    def state(self, b: int, p: int) -> np.ndarray:
+        """interior state (nc, ni, nj, nk) of family p in block b"""
+        B = self.case.blocks[b - 1]
+        nc = NCOND[self.case.fams[p - 1].closure]
+        ni, nj, nk = B.dims
+        out = np.empty(nc * (ni + 4) * (nj + 4) * (nk + 4))
+        self.check(self.f("get_state")(self.ctx, b, p, out), "get_state")
+        a = out.reshape((nc, ni + 4, nj + 4, nk + 4), order="F")
+        return a[:, 2:ni + 2, 2:nj + 2, 2:nk + 2].copy()
+
+    def bc_maps(self, p: int):
+        nrec = self.case.reci.shape[0]
+        partner = np.zeros(nrec, np.int32)
+        owner = np.zeros(nrec, np.int32)
+        cf = np.zeros(nrec, np.int32)
+        cap = nrec + 2
+        first = np.zeros(cap, np.int32)
+        rec = np.zeros(cap, np.int32)
+        ncell = self.C.c_int32()
+        self.check(self.f("get_bc_maps")(self.ctx, partner, owner, cf, p, self.C.byref(ncell), first, rec, cap),
+                   "get_bc_maps")
+        n = ncell.value
+        return partner, owner, cf, first[:n + 1].copy(), rec[:first[n] - 1].copy()
+
+    def close(self):
+        if self.ctx:
+            self.f("destroy")(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

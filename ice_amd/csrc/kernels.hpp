@@ -1,0 +1,168 @@
+// Kernels of one RK stage of ICE's explicit path (Mod_Explicit.f90:105-177) on
+// gfx950.  The state of a family lives in HBM as structure-of-arrays: slot v of
+// padded cell g at prim[v * N + g], where N is the padded cell count of all
+// blocks and block b's cell (i, j, k), Fortran 1-based with i in [-1, ni+2],
+// is g = off_b + (i+1) + NI*((j+1) + NJ*(k+1)), NI = ni+4, NJ = nj+4.
+// Geometry and gas use the same index, so every stream is unit-stride in i.
+#pragma once
+#include "dev_physics.hpp"
+
+namespace ice {
+
+constexpr int kGuide[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+struct BlkDev {
+  int ni, nj, nk, NI, NJ, NK;
+  long long off;   // padded offset of the block
+  long long cells; // ni*nj*nk
+};
+
+// A boundary record (ICE_bc_type) with its pairing
+struct DRec {
+  int b, i, j, k, f, type, p, bs, is, js, ks, fs, normal_dir, nchi1, nchi2, donor0, partner, owner, cf, pad;
+  double massflux, velocity, temperature, radius, alpha, beta, gfixed, pad2;
+};
+
+struct DevStatus {
+  unsigned long long dtbits;  // running min of the step's dt (positive doubles order as their bits)
+  double dtglobal, time;
+  unsigned long long n_floored;
+  int bad_state, bad_recon;
+  int bad_b, bad_i, bad_j, bad_k, bad_p;
+  int pad;
+};
+
+// Everything a kernel of one family reads or writes
+struct FamArgs {
+  const BlkDev* blk;
+  long long N;
+  // geometry
+  const double* vol;    // [N]
+  const double* dl;     // [3][N]
+  const double* M;      // [9][N] row-major entries
+  const double* fa;     // [3 dirs][4][N]: N1 N2 N3 A of the face above the cell in each direction
+  // gas
+  const double* gas;    // [5][N]
+  const double* gR;
+  const double* gG;
+  const double* gK;
+  const double* gMu;
+  const double* yinf;
+  // state
+  const double* cur;    // [nc][N]
+  const double* old;
+  double* nxt;
+  double* prim_w;       // ghost kernels write the current buffer through this
+  double* beta;         // [N]
+  double* dt;           // [N] steady runs
+  double* tau;          // [N]
+  uint8_t* inj;         // [N]
+  double* src;          // [nc][N] kept source (optional)
+  double* R;            // [nc][N] materialised residual (IRS)
+  double* T;            // [nc][N] IRS sweep buffer
+  double* bsum;         // [nbcell][nc] boundary-flux sums
+  const int* bidx;      // [N] boundary cell of a surface cell, -1 none
+  double* connF;        // [nconn][nc]
+  // records
+  const DRec* rec;
+  const int* donor;     // [ndonor][4]
+  const double* weight;
+  const int* list;      // record list of the launch (family records / owner records)
+  int nlist;
+  const int* bfirst;    // boundary cells: records bfirst[c]..bfirst[c+1]-1 of brec (0-based)
+  const int* brec;
+  const long long* bcell_g;  // padded index of each boundary cell
+  int nbcell;
+  DevStatus* st;
+  MatDev mat;
+  int limiter, nrk, srk, time_accurate, coupled, drag_sel, heat_sel, blow_sel;
+  int nfrozen, frozen[4];
+  int keep_source, last_stage, var_yinf, iter, cfl_ramp_iter, ndir, fam;
+  double cfl, dt_max, tau_factor, irs_beta;
+};
+
+ICE_DEV long long gidx(const BlkDev& B, int i, int j, int k) {
+  return B.off + (long long)(i + 1) + (long long)B.NI * ((long long)(j + 1) + (long long)B.NJ * (long long)(k + 1));
+}
+
+template <int NC> ICE_DEV void load_cell(const double* p, long long N, long long g, double* q) {
+#pragma unroll
+  for (int v = 0; v < NC; ++v) q[v] = p[(long long)v * N + g];
+}
+template <int NC> ICE_DEV void store_cell(double* p, long long N, long long g, const double* q) {
+#pragma unroll
+  for (int v = 0; v < NC; ++v) p[(long long)v * N + g] = q[v];
+}
+
+// ------------------------------------------------------------ interior face
+// Face between padded cell g and g + stride (direction d), its lower cell at
+// index m (1-based) of n along d.  Mod_Fluxes.f90:599-685.
+template <int M, int S, bool SD>
+ICE_DEV bool face_flux(const FamArgs& a, long long g, long long stride, int d, int m, int n, double* fl) {
+  constexpr int NC = Closure<M>::NC;
+  const long long N = a.N;
+  double st[4][NC];
+  double len[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    long long gs = g + (long long)(s - 1) * stride;
+    load_cell<NC>(a.cur, N, gs, st[s]);
+    len[s] = a.dl[(long long)d * N + gs];
+  }
+  double bf = SD ? dmin2(a.beta[g], a.beta[g + stride]) : 1.0;
+  if (m == 1 && a.inj[g - stride]) len[0] = 0.0;
+  if (m == n - 1 && a.inj[g + 2 * stride]) len[3] = 0.0;
+  const double* fa = a.fa + (long long)(d * 4) * N;
+  double nrm[3] = {fa[g], fa[N + g], fa[2 * N + g]};
+  double area = fa[3 * N + g];
+  double dl0 = 0.5 * (len[0] + len[1]);
+  double dl1 = 0.5 * (len[1] + len[2]);
+  double dl2 = 0.5 * (len[2] + len[3]);
+  double dll = 0.5 * len[1];
+  double dlr = 0.5 * len[2];
+  double ql[NC], qr[NC];
+  bool ok = reconstruct<M>(a.limiter, st[0], st[1], st[2], st[3], dl0, dl1, dl2, dll, dlr, bf, ql, qr);
+  riemann<M, S>(ql, qr, nrm, fl, a.mat);
+#pragma unroll
+  for (int v = 0; v < NC; ++v) fl[v] = fl[v] * area;
+  return ok;
+}
+
+// ------------------------------------------------------------ cell source
+// compute_source_ (Mod_Sources.f90:465-622) with cell_surface (:230-274), the
+// ITC surface (Lib_Drop_Surface.f90:15-43), no combustion, no P2T/boiling.
+template <int M>
+ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double* src, double& tau) {
+  constexpr int NC = Closure<M>::NC;
+  const long long N = a.N;
+  double gp[5];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) gp[m] = a.gas[(long long)m * N + g];
+  double R = a.gR[g], gam = a.gG[g], gk = a.gK[g], mu = a.gMu[g];
+  const MatDev& mt = a.mat;
+  double rho_mat = mt.rho;
+  double Rp = pow(0.75 * q[0] / (q[NC - 1] * kPi * rho_mat), 1.0 / 3.0);
+  double slip = norm3(gp[1] - q[1], gp[2] - q[2], gp[3] - q[3]);
+  double Re = 2.0 * gp[0] * Rp * slip / mu;
+  double Ma = slip / sqrt(gam * R * gp[4]);
+  double Pr = gam / (gam - 1.0) * R * mu / gk;
+  double Nu = nusselt(Re, Pr, Ma, a.heat_sel);
+  double Ts = q[NC - 2];
+  double Tr = Ts / gp[4];
+  double Cd = drag_cd(Re, Ma, gam, Tr, a.drag_sel);
+  double F[6];
+  F[1] = gp[1] - q[1];
+  F[2] = gp[2] - q[2];
+  F[3] = gp[3] - q[3];
+  double Qconv = 2.0 * Nu * gk * kPi * Rp * (gp[4] - Ts);
+  double mdot = 0.0;
+  F[0] = -mdot * q[NC - 1];
+  F[4] = Qconv * q[NC - 1];
+  F[4] = F[4] + mt.emiss * kSigmaSB * 4.0 * kPi * Rp * Rp * q[NC - 1] * (pow(gp[4], 4.0) - pow(Ts, 4.0));
+  if (Cd == 0.0) tau = __longlong_as_double(0x7FF0000000000000ll);
+  else tau = 8.0 * rho_mat * Rp / (3.0 * gp[0] * Cd * slip + kTauToll);
+  F[5] = tau;
+  sourcemake<M>(q, F, src, mt);
+}
+
+}  // namespace ice

@@ -1,0 +1,609 @@
+// The kernels of one family's stage, plain path (one launch per block for the
+// per-cell kernels, one per family for the per-record kernels).
+#pragma once
+#include "kernels.hpp"
+
+namespace ice {
+
+constexpr int kThreads = 256;
+
+ICE_DEV void cell_of(const BlkDev& B, long long t, int& i, int& j, int& k) {
+  i = (int)(t % B.ni) + 1;
+  long long r = t / B.ni;
+  j = (int)(r % B.nj) + 1;
+  k = (int)(r / B.nj) + 1;
+}
+
+ICE_DEV long long dir_stride(const BlkDev& B, int d) {
+  return d == 0 ? 1ll : (d == 1 ? (long long)B.NI : (long long)B.NI * (long long)B.NJ);
+}
+
+// block-wide minimum of a positive double, then one atomic per workgroup
+ICE_DEV void block_min_atomic(double v, unsigned long long* target) {
+  __shared__ double wmin[kThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) v = dmin2(v, __shfl_xor(v, o, 64));
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) wmin[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmin[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = dmin2(m, wmin[q]);
+    atomicMin(target, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ------------------------------------------------------------------ dt
+// compute_dt / compute_dt_ (Mod_dt.f90:11-112, 256-275)
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_dt(FamArgs a, int b) {
+  constexpr int NC = Closure<M>::NC;
+  const BlkDev B = a.blk[b];
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double dtc = 1.79769313486231571e308;
+  if (t < B.cells) {
+    int i, j, k;
+    cell_of(B, t, i, j, k);
+    long long g = gidx(B, i, j, k);
+    double q[NC];
+    load_cell<NC>(a.cur, a.N, g, q);
+    double dt = 1e+5;
+    double rate = 0.0;
+    for (int d = 0; d < a.ndir; ++d) {
+      double r0 = a.M[(long long)(d * 3 + 0) * a.N + g];
+      double r1 = a.M[(long long)(d * 3 + 1) * a.N + g];
+      double r2 = a.M[(long long)(d * 3 + 2) * a.N + g];
+      double gn = norm3(r0, r1, r2);
+      double vs[3] = {r0 / gn, r1 / gn, r2 / gn};
+      double un = q[1] * vs[0] + q[2] * vs[1] + q[3] * vs[2];
+      rate = rate + (fabs(un) + wavespeed<M>(q, vs)) * gn;
+    }
+    if (rate > 0.0) dt = dmin2(dt, 1.0 / rate);
+    dt = dt * a.cfl;
+    if (a.iter <= a.cfl_ramp_iter) dt = dt * (double)a.iter / (double)a.cfl_ramp_iter;
+    dt = dmin2(dt, a.dt_max);
+    if (a.coupled && a.tau_factor > 0.0) {
+      double tau = a.tau[g];
+      if (q[0] > kRhoEmpty && !nonfinite(tau) && tau > 0.0) dt = dmin2(dt, a.tau_factor * tau);
+    }
+    if (!a.time_accurate) a.dt[g] = dt;
+    dtc = dt;
+  }
+  block_min_atomic(dtc, &a.st->dtbits);
+}
+
+__global__ void k_step_begin(DevStatus* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) st->dtbits = (unsigned long long)__double_as_longlong(1e+5);
+}
+
+__global__ void k_dt_end(DevStatus* st, int time_accurate) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double dt = __longlong_as_double((long long)st->dtbits);
+    st->dtglobal = dt;
+    if (time_accurate) st->time = st->time + dt;
+  }
+}
+
+// ------------------------------------------------------------------ ghosts
+enum { kCopy = 0, kMirror = 1, kInject = 2 };
+
+ICE_DEV int ghost_rule(const DRec& r, double veln) {
+  int rule = kCopy;
+  if (r.type == 200) rule = kMirror;
+  else if (r.type == 300 || r.type == 301) {
+    if (veln * (double)(1 - 2 * (r.f % 2)) <= 0.0 || r.f == 3) rule = kMirror;
+  } else if (r.type >= 401 && r.type <= 403) {
+    if (!(veln * (double)(1 - 2 * (r.f % 2)) > 0.0)) {
+      rule = kInject;
+      if (r.massflux == 0.0) rule = kMirror;
+    }
+  }
+  return rule;
+}
+
+// normal of the boundary face of a record (Lib_Ghost.f90:111-124)
+ICE_DEV void bnd_normal(const FamArgs& a, const BlkDev& B, const DRec& r, double* nrm) {
+  int i = r.i, j = r.j, k = r.k, d;
+  if (r.f <= 2) { i -= r.f % 2; d = 0; }
+  else if (r.f <= 4) { j -= r.f % 2; d = 1; }
+  else { k -= r.f % 2; d = 2; }
+  long long g = gidx(B, i, j, k);
+  const double* fa = a.fa + (long long)(d * 4) * a.N;
+  nrm[0] = fa[g];
+  nrm[1] = fa[a.N + g];
+  nrm[2] = fa[2 * a.N + g];
+}
+
+// conservative blend of chimera donors into ghost layer lay (Lib_Ghost.f90:439-466)
+template <int M> ICE_DEV void chimera(const FamArgs& a, const DRec& r, int lay) {
+  constexpr int NC = Closure<M>::NC;
+  int c1 = lay == 1 ? 1 : r.nchi1 + 1;
+  int c2 = lay == 1 ? r.nchi1 : r.nchi1 + r.nchi2;
+  double cg[NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) cg[v] = 0.0;
+  for (int c = c1; c <= c2; ++c) {
+    const int* d = a.donor + (long long)(r.donor0 + c - 1) * 4;
+    const BlkDev Bd = a.blk[d[0] - 1];
+    double q[NC], cs[NC];
+    load_cell<NC>(a.prim_w, a.N, gidx(Bd, d[1], d[2], d[3]), q);
+    prim2cons<M>(q, cs, a.mat);
+    double w = a.weight[r.donor0 + c - 1];
+#pragma unroll
+    for (int v = 0; v < NC; ++v) cg[v] = cg[v] + w * cs[v];
+  }
+  const BlkDev B = a.blk[r.b - 1];
+  const int* G = kGuide[r.f - 1];
+  double q[NC];
+  cons2prim<M>(cg, q, a.mat);
+  store_cell<NC>(a.prim_w, a.N, gidx(B, r.i - lay * G[0], r.j - lay * G[1], r.k - lay * G[2]), q);
+}
+
+// first ghost layer (ghost_record, Lib_Ghost.f90:76-267)
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_ghost1(FamArgs a) {
+  constexpr int NC = Closure<M>::NC;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nlist) return;
+  const DRec r = a.rec[a.list[t]];
+  const BlkDev B = a.blk[r.b - 1];
+  const long long N = a.N;
+  const int* G = kGuide[r.f - 1];
+  long long gg = gidx(B, r.i - G[0], r.j - G[1], r.k - G[2]);
+  long long gm = gidx(B, r.i, r.j, r.k);
+  double* P = a.prim_w;
+  double in[NC], gh[NC];
+  switch (r.type) {
+    case 101:
+    case 201: {
+      const BlkDev Bs = a.blk[r.bs - 1];
+      load_cell<NC>(P, N, gidx(Bs, r.is, r.js, r.ks), gh);
+      store_cell<NC>(P, N, gg, gh);
+      break;
+    }
+    case 200:
+    case 300:
+    case 301: {
+      double nrm[3];
+      bnd_normal(a, B, r, nrm);
+      load_cell<NC>(P, N, gm, in);
+#pragma unroll
+      for (int v = 0; v < NC; ++v) gh[v] = in[v];
+      double vel[3] = {in[1], in[2], in[3]};
+      double veln = vel[0] * nrm[0] + vel[1] * nrm[1] + vel[2] * nrm[2];
+      if (ghost_rule(r, veln) == kMirror) {
+        for (int m = 0; m < 3; ++m) vel[m] = vel[m] - 2.0 * veln * nrm[m];
+        if constexpr (M == 2) mirror6(&in[4], nrm, &gh[4]);
+      }
+      gh[1] = vel[0];
+      gh[2] = vel[1];
+      gh[3] = vel[2];
+      store_cell<NC>(P, N, gg, gh);
+      break;
+    }
+    case 0:
+    case 400:
+      load_cell<NC>(P, N, gm, gh);
+      store_cell<NC>(P, N, gg, gh);
+      break;
+    case 401:
+    case 402:
+    case 403: {
+      double nrm[3];
+      bnd_normal(a, B, r, nrm);
+      load_cell<NC>(P, N, gm, in);
+      double vel[3] = {in[1], in[2], in[3]};
+      double veln = vel[0] * nrm[0] + vel[1] * nrm[1] + vel[2] * nrm[2];
+      int rule = ghost_rule(r, veln);
+      a.inj[gg] = (rule == kInject);
+      if (rule == kCopy) {
+        store_cell<NC>(P, N, gg, in);
+        break;
+      }
+      if (rule == kMirror) {
+#pragma unroll
+        for (int v = 0; v < NC; ++v) gh[v] = in[v];
+        for (int m = 0; m < 3; ++m) vel[m] = vel[m] - 2.0 * veln * nrm[m];
+        gh[1] = vel[0];
+        gh[2] = vel[1];
+        gh[3] = vel[2];
+        if constexpr (M == 2) mirror6(&in[4], nrm, &gh[4]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < NC; ++v) gh[v] = 0.0;
+        double sg = (double)(2 * (r.f % 2) - 1);
+        double inward[3] = {sg * nrm[0], sg * nrm[1], sg * nrm[2]};
+        double spd;
+        if (r.type == 401) {
+          double vg = a.gas[1 * N + gm] * inward[0] + a.gas[2 * N + gm] * inward[1] + a.gas[3 * N + gm] * inward[2];
+          spd = r.velocity * vg;
+        } else if (r.type == 402) {
+          spd = r.velocity;
+        } else {
+          spd = r.velocity * norm3(a.gas[1 * N + gm], a.gas[2 * N + gm], a.gas[3 * N + gm]);
+        }
+        if (r.normal_dir) {
+          gh[1] = spd * inward[0];
+          gh[2] = spd * inward[1];
+          gh[3] = spd * inward[2];
+        } else {
+          gh[1] = spd * cos(r.beta) * cos(r.alpha);
+          gh[2] = spd * cos(r.beta) * sin(r.alpha);
+          gh[3] = spd * sin(r.beta);
+        }
+        if (r.type == 401) {
+          gh[0] = r.massflux * a.gas[gm] / r.velocity;
+          double vn = fabs(gh[1] * nrm[0] + gh[2] * nrm[1] + gh[3] * nrm[2]);
+          if (r.gfixed > 0.0 && vn > 0.0) gh[0] = gh[0] + r.massflux * r.gfixed / vn;
+          gh[NC - 2] = r.temperature * a.gas[4 * N + gm];
+        } else {
+          double vn = gh[1] * nrm[0] + gh[2] * nrm[1] + gh[3] * nrm[2];
+          gh[0] = r.massflux / fabs(vn);
+          gh[NC - 2] = r.temperature;
+        }
+        gh[NC - 1] = gh[0] / a.mat.rho / (4.0 / 3.0 * kPi * (r.radius * r.radius * r.radius));
+        if constexpr (M == 1) gh[4] = 1e-6;
+        if constexpr (M == 2) {
+          gh[4] = 1e-6;
+          gh[7] = 1e-6;
+          gh[9] = 1e-6;
+        }
+      }
+      store_cell<NC>(P, N, gg, gh);
+      break;
+    }
+    case 102:
+      chimera<M>(a, r, 1);
+      break;
+    default:
+      break;
+  }
+}
+
+// second ghost layer (second_ghost_record, Lib_Ghost.f90:312-387)
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_ghost2(FamArgs a) {
+  constexpr int NC = Closure<M>::NC;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nlist) return;
+  const DRec r = a.rec[a.list[t]];
+  if (r.type == 0) return;
+  const BlkDev B = a.blk[r.b - 1];
+  const long long N = a.N;
+  const int* G = kGuide[r.f - 1];
+  long long g1 = gidx(B, r.i - G[0], r.j - G[1], r.k - G[2]);
+  long long g2 = gidx(B, r.i - 2 * G[0], r.j - 2 * G[1], r.k - 2 * G[2]);
+  long long gm = gidx(B, r.i, r.j, r.k);
+  long long gp = gidx(B, r.i + G[0], r.j + G[1], r.k + G[2]);
+  double* P = a.prim_w;
+  double q[NC];
+  switch (r.type) {
+    case 101:
+    case 201: {
+      const BlkDev Bs = a.blk[r.bs - 1];
+      const int* Gs = kGuide[r.fs - 1];
+      load_cell<NC>(P, N, gidx(Bs, r.is + Gs[0], r.js + Gs[1], r.ks + Gs[2]), q);
+      store_cell<NC>(P, N, g2, q);
+      break;
+    }
+    case 102:
+      chimera<M>(a, r, 2);
+      break;
+    case 200:
+    case 300:
+    case 301:
+    case 401:
+    case 402:
+    case 403: {
+      double nrm[3];
+      bnd_normal(a, B, r, nrm);
+      double veln = P[1 * N + gm] * nrm[0] + P[2 * N + gm] * nrm[1] + P[3 * N + gm] * nrm[2];
+      int rule = ghost_rule(r, veln);
+      if (rule == kInject) {
+        load_cell<NC>(P, N, g1, q);
+        store_cell<NC>(P, N, g2, q);
+      } else {
+        load_cell<NC>(P, N, gp, q);
+        double o[NC];
+#pragma unroll
+        for (int v = 0; v < NC; ++v) o[v] = q[v];
+        if (rule == kMirror) {
+          double d = q[1] * nrm[0] + q[2] * nrm[1] + q[3] * nrm[2];
+          for (int m = 0; m < 3; ++m) o[1 + m] = q[1 + m] - 2.0 * d * nrm[m];
+          if constexpr (M == 2) mirror6(&q[4], nrm, &o[4]);
+        }
+        store_cell<NC>(P, N, g2, o);
+      }
+      break;
+    }
+    default: {
+      double c1[NC], cm[NC];
+      load_cell<NC>(P, N, g1, c1);
+      load_cell<NC>(P, N, gm, cm);
+      load_cell<NC>(P, N, gp, q);
+#pragma unroll
+      for (int v = 0; v < NC; ++v) q[v] = 3.0 * c1[v] - 3.0 * cm[v] + q[v];
+      store_cell<NC>(P, N, g2, q);
+      break;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ beta
+// tile_beta / SD_rho (Mod_Fluxes.f90:366-389)
+__global__ void __launch_bounds__(kThreads) k_beta(FamArgs a, int b) {
+  const BlkDev B = a.blk[b];
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.cells) return;
+  int i, j, k;
+  cell_of(B, t, i, j, k);
+  long long g = gidx(B, i, j, k);
+  long long sj = B.NI, sk = (long long)B.NI * B.NJ;
+  const double* r = a.cur;
+  a.beta[g] = sd_weight(r[g], r[g - 1], r[g + 1], r[g - sj], r[g + sj], r[g - sk], r[g + sk]);
+}
+
+// ------------------------------------------------------------------ boundary fluxes
+// record_flux (Mod_BC_Fluxes.f90:180-272)
+template <int M, int S>
+ICE_DEV bool record_flux(const FamArgs& a, const DRec& r, double beta_val, double* flux) {
+  constexpr int NC = Closure<M>::NC;
+  const BlkDev B = a.blk[r.b - 1];
+  const long long N = a.N;
+  const int* G = kGuide[r.f - 1];
+  int dir = (r.f + 1) / 2 - 1;
+  long long gm = gidx(B, r.i, r.j, r.k);
+  long long g1 = gidx(B, r.i - G[0], r.j - G[1], r.k - G[2]);
+  long long g2 = gidx(B, r.i - 2 * G[0], r.j - 2 * G[1], r.k - 2 * G[2]);
+  long long gp = gidx(B, r.i + G[0], r.j + G[1], r.k + G[2]);
+  double dl_g1 = a.dl[(long long)dir * N + g1];
+  double dl_m = a.dl[(long long)dir * N + gm];
+  double dl_4 = a.dl[(long long)dir * N + gp];
+  bool inj_g = a.inj[g1], inj_p = a.inj[gp];
+  double s0[NC], s1[NC], s2[NC], s3[NC], ql[NC], qr[NC], fl[NC];
+  const double* fa = a.fa + (long long)(dir * 4) * N;
+  bool ok;
+  if (r.f % 2 == 1) {
+    long long gf = g1;
+    double nrm[3] = {fa[gf], fa[N + gf], fa[2 * N + gf]};
+    double area = fa[3 * N + gf];
+    double dl0 = dl_g1, dl1 = 0.5 * (dl_g1 + dl_m), dl2 = 0.5 * (dl_m + dl_4);
+    double dll = 0.5 * dl_g1, dlr = 0.5 * dl_m;
+    if (inj_g) { dl1 = 0.5 * dl_m; dll = 0.0; }
+    if (inj_p) dl2 = 0.5 * dl_m;
+    load_cell<NC>(a.cur, N, g2, s0);
+    load_cell<NC>(a.cur, N, g1, s1);
+    load_cell<NC>(a.cur, N, gm, s2);
+    load_cell<NC>(a.cur, N, gp, s3);
+    ok = reconstruct<M>(a.limiter, s0, s1, s2, s3, dl0, dl1, dl2, dll, dlr, beta_val, ql, qr);
+    riemann<M, S>(ql, qr, nrm, fl, a.mat);
+#pragma unroll
+    for (int v = 0; v < NC; ++v) flux[v] = fl[v] * area;
+  } else {
+    long long gf = gm;
+    double nrm[3] = {fa[gf], fa[N + gf], fa[2 * N + gf]};
+    double area = fa[3 * N + gf];
+    double dl0 = 0.5 * (dl_4 + dl_m), dl1 = 0.5 * (dl_m + dl_g1), dl2 = dl_g1;
+    double dll = 0.5 * dl_m, dlr = 0.5 * dl_g1;
+    if (inj_g) { dl1 = 0.5 * dl_m; dlr = 0.0; }
+    if (inj_p) dl0 = 0.5 * dl_m;
+    load_cell<NC>(a.cur, N, gp, s0);
+    load_cell<NC>(a.cur, N, gm, s1);
+    load_cell<NC>(a.cur, N, g1, s2);
+    load_cell<NC>(a.cur, N, g2, s3);
+    ok = reconstruct<M>(a.limiter, s0, s1, s2, s3, dl0, dl1, dl2, dll, dlr, beta_val, ql, qr);
+    riemann<M, S>(ql, qr, nrm, fl, a.mat);
+#pragma unroll
+    for (int v = 0; v < NC; ++v) flux[v] = -(fl[v] * area);
+  }
+  return ok;
+}
+
+// owner_flux (Mod_BC_Fluxes.f90:151-170): one thread per owner record
+template <int M, int S, bool SD>
+__global__ void __launch_bounds__(kThreads) k_conn(FamArgs a) {
+  constexpr int NC = Closure<M>::NC;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nlist) return;
+  const DRec r = a.rec[a.list[t]];
+  double bf = 1.0;
+  if (SD) {
+    const DRec q = a.rec[r.partner - 1];
+    double bq = a.beta[gidx(a.blk[q.b - 1], q.i, q.j, q.k)];
+    double bn = a.beta[gidx(a.blk[r.b - 1], r.i, r.j, r.k)];
+    bf = dmin2(bn, bq);
+  }
+  double fl[NC];
+  if (!record_flux<M, S>(a, r, bf, fl)) atomicOr(&a.st->bad_recon, 1);
+#pragma unroll
+  for (int v = 0; v < NC; ++v) a.connF[(long long)(r.cf - 1) * NC + v] = fl[v];
+}
+
+// bound_cell (Mod_BC_Fluxes.f90:105-144): one thread per boundary cell, its
+// records in face order
+template <int M, int S, bool SD>
+__global__ void __launch_bounds__(kThreads) k_bcell(FamArgs a) {
+  constexpr int NC = Closure<M>::NC;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.nbcell) return;
+  double acc[NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) acc[v] = 0.0;
+  bool ok = true;
+  for (int rr = a.bfirst[c]; rr < a.bfirst[c + 1]; ++rr) {
+    const DRec r = a.rec[a.brec[rr]];
+    if (r.type == 200 && M == 0) continue;
+    double fl[NC];
+    if (r.type == 101 || r.type == 201) {
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        double x = a.connF[(long long)(r.cf - 1) * NC + v];
+        fl[v] = r.owner ? x : -x;
+      }
+    } else {
+      double bv = SD ? a.beta[gidx(a.blk[r.b - 1], r.i, r.j, r.k)] : 1.0;
+      ok = record_flux<M, S>(a, r, bv, fl) && ok;
+    }
+#pragma unroll
+    for (int v = 0; v < NC; ++v) acc[v] = acc[v] + fl[v];
+  }
+  if (!ok) atomicOr(&a.st->bad_recon, 1);
+#pragma unroll
+  for (int v = 0; v < NC; ++v) a.bsum[(long long)c * NC + v] = acc[v];
+}
+
+// ------------------------------------------------------------------ update
+// state_update_ (Lib_Newstate.f90:152-173) of one cell
+template <int M>
+ICE_DEV void update_cell(const FamArgs& a, const BlkDev& B, int i, int j, int k, long long g, const double* R) {
+  constexpr int NC = Closure<M>::NC;
+  double q[NC], co[NC], cu[NC];
+  load_cell<NC>(a.old, a.N, g, q);
+  prim2cons<M>(q, co, a.mat);
+  load_cell<NC>(a.cur, a.N, g, q);
+  prim2cons<M>(q, cu, a.mat);
+  double w = rk_weight(a.nrk, a.srk);
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    double rr = (cu[v] - co[v] + R[v]) * w;
+    cu[v] = co[v] + rr;
+  }
+  if (cu[0] < 0.0) atomicAdd(&a.st->n_floored, 1ull);
+  cons2prim<M>(cu, q, a.mat);
+  store_cell<NC>(a.nxt, a.N, g, q);
+  if (!checkprim<M>(q)) {
+    if (atomicCAS(&a.st->bad_state, 0, 1) == 0) {
+      int bidx = (int)(&B - a.blk);
+      a.st->bad_b = bidx + 1;
+      a.st->bad_i = i;
+      a.st->bad_j = j;
+      a.st->bad_k = k;
+      a.st->bad_p = a.fam + 1;
+    }
+  }
+}
+
+// The cell's stage: boundary sum, interior faces in the reference's order
+// (Mod_Fluxes.f90:471-508), source, residual (Lib_Residual.f90:147-156) and
+// either the RK update (WRITE_R = false) or the residual for smoothing.
+template <int M, int S, bool SD, bool COUPLED, bool WRITE_R>
+__global__ void __launch_bounds__(kThreads) k_stage(FamArgs a, int b) {
+  constexpr int NC = Closure<M>::NC;
+  const BlkDev& B = a.blk[b];
+  const BlkDev Bv = B;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Bv.cells) return;
+  int i, j, k;
+  cell_of(Bv, t, i, j, k);
+  long long g = gidx(Bv, i, j, k);
+  double r[NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) r[v] = 0.0;
+  bool surface = i == 1 || i == Bv.ni || j == 1 || j == Bv.nj || k == 1 || k == Bv.nk;
+  if (surface) {
+    int c = a.bidx[g];
+    if (c >= 0) {
+#pragma unroll
+      for (int v = 0; v < NC; ++v) r[v] = a.bsum[(long long)c * NC + v];
+    }
+  }
+  bool ok = true;
+  const int idx[3] = {i, j, k};
+  const int nd[3] = {Bv.ni, Bv.nj, Bv.nk};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int n = nd[d], m = idx[d];
+    if (n < 2) continue;
+    long long s = dir_stride(Bv, d);
+    double f[NC];
+    if (m % 2 == 1) {
+      if (m < n) {
+        ok = face_flux<M, S, SD>(a, g, s, d, m, n, f) && ok;
+#pragma unroll
+        for (int v = 0; v < NC; ++v) r[v] = r[v] - f[v];
+      }
+      if (m > 1) {
+        ok = face_flux<M, S, SD>(a, g - s, s, d, m - 1, n, f) && ok;
+#pragma unroll
+        for (int v = 0; v < NC; ++v) r[v] = r[v] + f[v];
+      }
+    } else {
+      ok = face_flux<M, S, SD>(a, g - s, s, d, m - 1, n, f) && ok;
+#pragma unroll
+      for (int v = 0; v < NC; ++v) r[v] = r[v] + f[v];
+      if (m < n) {
+        ok = face_flux<M, S, SD>(a, g, s, d, m, n, f) && ok;
+#pragma unroll
+        for (int v = 0; v < NC; ++v) r[v] = r[v] - f[v];
+      }
+    }
+  }
+  if (!ok) atomicOr(&a.st->bad_recon, 1);
+  double src[NC];
+  if constexpr (COUPLED) {
+    double q[NC], tau;
+    load_cell<NC>(a.cur, a.N, g, q);
+    cell_source<M>(a, g, q, src, tau);
+    for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
+    if (a.last_stage) {
+      a.tau[g] = tau;
+      if (a.keep_source) store_cell<NC>(a.src, a.N, g, src);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < NC; ++v) src[v] = 0.0;
+  }
+  double vol = a.vol[g];
+  double dt = a.time_accurate ? a.st->dtglobal : a.dt[g];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) r[v] = (r[v] / vol + src[v]) * dt;
+  for (int f = 0; f < a.nfrozen; ++f) r[a.frozen[f] - 1] = 0.0;
+  if constexpr (WRITE_R) {
+    store_cell<NC>(a.R, a.N, g, r);
+  } else {
+    update_cell<M>(a, B, i, j, k, g, r);
+  }
+}
+
+// ------------------------------------------------------------------ IRS
+// residual_smoothing_ (Lib_IRS.f90:34-86), direction d: the first Jacobi sweep
+// into T, the second back into R; ghosts are the zero-gradient copies of the
+// residual the direction started from.
+template <int NC>
+__global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int b, int d, int sweep) {
+  const BlkDev B = a.blk[b];
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.cells) return;
+  int i, j, k;
+  cell_of(B, t, i, j, k);
+  long long g = gidx(B, i, j, k);
+  long long s = dir_stride(B, d);
+  int m = d == 0 ? i : (d == 1 ? j : k);
+  int n = d == 0 ? B.ni : (d == 1 ? B.nj : B.nk);
+  const double* star = sweep == 1 ? a.R : a.T;
+  double* out = sweep == 1 ? a.T : a.R;
+  double den = 1.0 + 2.0 * a.irs_beta;
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    long long o = (long long)v * a.N;
+    double r0 = a.R[o + g];
+    double lo = m > 1 ? star[o + g - s] : r0;
+    double hi = m < n ? star[o + g + s] : r0;
+    out[o + g] = (r0 + a.irs_beta * (lo + hi)) / den;
+  }
+}
+
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_update(FamArgs a, int b) {
+  constexpr int NC = Closure<M>::NC;
+  const BlkDev& B = a.blk[b];
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.cells) return;
+  int i, j, k;
+  cell_of(B, t, i, j, k);
+  long long g = gidx(B, i, j, k);
+  double r[NC];
+  load_cell<NC>(a.R, a.N, g, r);
+  update_cell<M>(a, B, i, j, k, g, r);
+}
+
+}  // namespace ice

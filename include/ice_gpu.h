@@ -1,0 +1,168 @@
+/*
+ * ice_gpu.h -- C-ABI of the MI355X explicit residual-and-update path.
+ *
+ * The path replaces the body of ICE's `Explicit_Step(grid)`
+ * (src/lib/numerics/time/explicit/Mod_Explicit.f90:11-245), called once per
+ * iteration from `ICE_solve` (src/lib/driver/Wrap_Solve.f90:26).  The host keeps
+ * ICE's block / boundary-condition types (src/lib/base/Advanced_Types_m.f90:13-153)
+ * and hands their arrays over here by plain pointers in their Fortran layouts:
+ *
+ *   prim     (nc, 1-gc:ni+gc, 1-gc:nj+gc, 1-gc:nk+gc)   gc = 2   (Mod_Allocate_Data.f90:216-238)
+ *   vol      (ni, nj, nk)
+ *   dl       (3, 0:ni+1, 0:nj+1, 0:nk+1)
+ *   M        (3, 3, ni, nj, nk)                  metric tensor, row d = grad xi_d
+ *   face_i   (4, 0:ni, 1:nj, 1:nk)  {N(3), A}    ICE_f_metrics_type (Base_Types_m.f90:22-25)
+ *   face_j   (4, 1:ni, 0:nj, 1:nk)
+ *   face_k   (4, 1:ni, 1:nj, 0:nk)
+ *   gas prim (5, ni, nj, nk) = rho, u, v, w, T;  R, gam, k, mu (ni, nj, nk)
+ *
+ * Every call is synchronous at return, from one host thread.  Status 0 = ok;
+ * a non-zero status leaves a message for ice_gpu_last_error.  No torch types,
+ * no HIP types: a Fortran host binds these with ISO_C_BINDING (ice_gpu_m.f90),
+ * Python with ctypes.
+ */
+#ifndef ICE_GPU_H
+#define ICE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICE_GC        2    /* ghost layers, ICE_Global_m gc (Global_m.f90:9) */
+#define ICE_NCOND_MAX 14   /* Global_m.f90:24 */
+#define ICE_NEP       8    /* packed vapour properties, Lib_Evaporation.f90:15-19 */
+
+/* Closures (Lib_Model.f90:105-316).  nc = 6 / 7 / 12. */
+enum { ICE_MK = 0, ICE_IG = 1, ICE_AG = 2 };
+/* Riemann solvers (Lib_Riemann.f90:30-57) */
+enum { ICE_SAUREL = 0, ICE_RUSANOV = 1, ICE_HLLE = 2 };
+/* Limiters, in the order of assign_limiter (Lib_Limiters.f90:35-84) */
+enum { ICE_LIM_IORD = 0, ICE_LIM_MINMOD, ICE_LIM_VANALBADA, ICE_LIM_VANLEER, ICE_LIM_OSPRE,
+       ICE_LIM_UMIST, ICE_LIM_OSHER, ICE_LIM_SWEBY, ICE_LIM_MC, ICE_LIM_KOREN, ICE_LIM_SUPERBEE };
+/* Execution modes of ice_gpu_step */
+enum { ICE_MODE_PLAIN = 0, ICE_MODE_BATCHED = 1 };
+
+/* One particle family: its closure, its Riemann solver and its material
+ * (condensed_phase_t, Config_Types_m.f90:195-293, constant properties). */
+typedef struct {
+  int32_t closure;          /* ICE_MK / ICE_IG / ICE_AG                                  */
+  int32_t riemann;          /* ICE_SAUREL / ICE_RUSANOV / ICE_HLLE                       */
+  int32_t nfrozen;          /* slots not integrated (Mod_Freeze.f90:37-141)              */
+  int32_t frozen[4];        /* their 1-based slot numbers                                */
+  int32_t evap_select;      /* 0 none, 1 d2-law, 2 CEM, 3 CEM-B, 4 ASM, 5 TC             */
+  int32_t intf_select;      /* 0 VLE, 1 LK                                               */
+  int32_t surrogate;        /* 1: seed the LK interface solve with the learned model     */
+  double  rho_mat;          /* rho_al  [kg/m^3]                                          */
+  double  cs;               /* cs_al   [J/(kg K)]                                        */
+  double  lv;               /* lv_al   [J/kg]                                            */
+  double  emiss;            /* surface emissivity                                        */
+  double  ep[ICE_NEP];      /* Mv, Lv, cpv, Le, Yinf, Lv*Mv/Ru, 1/Tboil, alphaE          */
+} ice_family_t;
+
+/* time_scheme_t / space_scheme_t / irs_t subset (Config_Types_m.f90:119-180) */
+typedef struct {
+  int32_t ngroups;          /* particle families                                         */
+  int32_t nrk;              /* 1 euler, 2 RK2, 3 RK3                                     */
+  int32_t time_accurate;    /* 1: one global dt (min over cells, families)               */
+  int32_t cfl_ramp_iter;    /* cfl-rampa-iter                                            */
+  int32_t ndir;             /* resolved directions (setup_metrics)                       */
+  int32_t limiter;          /* ICE_LIM_*                                                 */
+  int32_t shock_detector;   /* 1: Jameson sensor weight on the slopes                    */
+  int32_t irs;              /* 1: implicit residual smoothing                            */
+  int32_t coupled;          /* one-way coupled to a frozen gas                           */
+  int32_t drag_select;      /* Lib_Drag selector 1..13                                   */
+  int32_t heat_select;      /* Lib_Heat selector 1..8                                    */
+  int32_t blow_select;      /* 0 none, 1 LK blowing factor                               */
+  int32_t mode;             /* ICE_MODE_PLAIN / ICE_MODE_BATCHED                         */
+  int32_t var_yinf;         /* 1: Yinf per cell from the gas (ice_gpu_put_gas_yinf)      */
+  int32_t reserved[2];
+  double  cfl, dt_max, tau_factor, irs_beta;
+} ice_config_t;
+
+/* Boundary records, one per (boundary cell, family), in the order of ICE's bc
+ * array (IO_BC.f90:299, idx = (n-1)*ngroups + p).  Integer fields, 1-based: */
+enum { ICE_R_B = 0, ICE_R_I, ICE_R_J, ICE_R_K, ICE_R_F, ICE_R_TYPE, ICE_R_P,
+       ICE_R_BS, ICE_R_IS, ICE_R_JS, ICE_R_KS, ICE_R_FS,
+       ICE_R_NORMAL_DIR, ICE_R_NCHI1, ICE_R_NCHI2, ICE_R_DONOR0, ICE_REC_NI };
+/* Real fields of an inlet record (ICE_bc_type, Advanced_Types_m.f90:111-118) */
+enum { ICE_RD_MASSFLUX = 0, ICE_RD_VELOCITY, ICE_RD_TEMPERATURE, ICE_RD_RADIUS,
+       ICE_RD_ALPHA, ICE_RD_BETA, ICE_RD_GFIXED, ICE_REC_ND };
+
+typedef struct {
+  int64_t n_floored;        /* update events with a negative density (Lib_Newstate.f90:178-193) */
+  int32_t bad_state;        /* a state failed check_prim after the update                        */
+  int32_t bad_block, bad_i, bad_j, bad_k, bad_p;   /* the first failing cell (1-based)           */
+  int32_t bad_recon;        /* an unphysical stencil at first order (Mod_Fluxes.f90:238-241)     */
+  int32_t iter;             /* grid%iter after the call                                          */
+  double  dtglobal;         /* last global dt (time-accurate)                                    */
+  double  time;             /* grid%time                                                         */
+} ice_status_t;
+
+typedef struct ice_gpu_ctx ice_gpu_ctx;
+
+/* Context on HIP device `device`: the run's configuration, its families and the
+ * cell counts of its nb blocks (dims = 3*nb ints).  Replaces the set-up that
+ * Wrap_Setup.f90:48-121 does for the hot path (allocate_data, setup_freeze). */
+int  ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams,
+                    int nb, const int32_t* dims, ice_gpu_ctx** ctx);
+
+/* Geometry of block b (1-based), from setup_metrics (Mod_Metrics.f90:19-59). */
+int  ice_gpu_set_geometry(ice_gpu_ctx* ctx, int b, const double* vol, const double* dl,
+                          const double* M, const double* face_i, const double* face_j,
+                          const double* face_k);
+
+/* The boundary table (Setup_BC, IO_BC.f90:65-344): nrec records of ICE_REC_NI
+ * ints and ICE_REC_ND reals; chimera donors (b,i,j,k) and weights, record r's
+ * first donor at reci[r*ICE_REC_NI+ICE_R_DONOR0] (0-based into the donor list).
+ * Pairs the connections (Pair_Connections, IO_BC.f90:360-510) and builds the
+ * per-family record lists and boundary cells (Mod_GhostExchange.f90:103-360). */
+int  ice_gpu_set_boundaries(ice_gpu_ctx* ctx, int nrec, const int32_t* reci, const double* recd,
+                            int ndonor, const int32_t* donor, const double* weight);
+
+/* The pairing it computed: partner record (1-based, 0 = none), owner flag and
+ * face number cf of every record; and family p's boundary cells, cell c holding
+ * records rec[first[c]-1 .. first[c+1]-2] (1-based, collect_boundary_cells). */
+int  ice_gpu_get_bc_maps(ice_gpu_ctx* ctx, int32_t* partner, int32_t* owner, int32_t* cf,
+                         int p, int32_t* ncell, int32_t* first, int32_t* rec, int cap);
+
+/* State of family p in block b, primitive variables with ghosts. */
+int  ice_gpu_put_state(ice_gpu_ctx* ctx, int b, int p, const double* prim);
+int  ice_gpu_get_state(ice_gpu_ctx* ctx, int b, int p, double* prim);
+
+/* Frozen gas of block b (set_gas, Mod_Exchange.f90:63-81; Mod_Phase.f90:11-63). */
+int  ice_gpu_put_gas(ice_gpu_ctx* ctx, int b, const double* gprim, const double* R,
+                     const double* gam, const double* k, const double* mu);
+/* Far-field vapour mass fraction per cell, replacing ep(iYinf) (cfg.var_yinf = 1). */
+int  ice_gpu_put_gas_yinf(ice_gpu_ctx* ctx, int b, const double* yinf);
+
+/* Relaxation time of family p (ni,nj,nk) before the first step; the stages write it. */
+int  ice_gpu_put_tau(ice_gpu_ctx* ctx, int b, int p, const double* tau);
+
+/* nsteps calls of Explicit_Step, all on the device; status after the last one.
+ * Returns non-zero when a step hit a fatal state (st->bad_state / bad_recon). */
+int  ice_gpu_step(ice_gpu_ctx* ctx, int nsteps, ice_status_t* st);
+
+/* 1: the last stage of every step keeps its source for ice_gpu_get_source. */
+int  ice_gpu_set_keep_source(ice_gpu_ctx* ctx, int on);
+
+/* Last stage's source (nc,ni,nj,nk) and relaxation time (ni,nj,nk) of family p. */
+int  ice_gpu_get_source(ice_gpu_ctx* ctx, int b, int p, double* source, double* tau);
+
+/* residual_norm (Mod_Diagnostic.f90:22-69) of the last step: total[5], fam[5*ngroups]. */
+int  ice_gpu_residual_norm(ice_gpu_ctx* ctx, double* total, double* fam);
+
+/* Time (ms, HIP events on the context's stream) of the last ice_gpu_step, and
+ * the summed duration and launch count of the dominant kernel in it. */
+int  ice_gpu_timing(ice_gpu_ctx* ctx, double* step_ms, double* kernel_ms, int64_t* kernel_launches);
+/* 1: bracket every launch of the dominant kernel with HIP events (bench only). */
+int  ice_gpu_set_kernel_timing(ice_gpu_ctx* ctx, int on);
+
+int  ice_gpu_last_error(ice_gpu_ctx* ctx, char* buf, int len);
+void ice_gpu_destroy(ice_gpu_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICE_GPU_H */
