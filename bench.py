@@ -1,0 +1,134 @@
+"""Throughput of the explicit residual-and-update path on one MI355X.
+
+A step is one Explicit_Step (all RK stages of every family) over the whole grid,
+with the state resident in HBM.  Metric: Mcell-stage updates/s = interior
+cells x families x RK stages x steps / time.  Prints one JSON line.
+
+Default workload: the 2-D crossing jets (1024 x 1024 cells, AG, HLLE, MUSCL van
+Leer with the Jameson sensor, drag + heat, SSP-RK3).  The CPU baseline is the
+OpenMP restatement of the same loops (oracle/liboracle.so) on a bounded sample
+of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from ice_amd import cases, native  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_kernel_bytes(case) -> float:
+    """Algorithmic HBM bytes per interior cell of one launch of the fused stage
+    kernel: read the stage state and the step's start state, write the new state
+    (3 nc), the geometry the faces and the cell read (1-D 6, 2-D 11, 3-D 16
+    doubles), and the frozen gas (9 doubles) when coupled; per family."""
+    geo = {1: 6, 2: 11, 3: 16}[case.cfg.ndir]
+    per = 0
+    for f in case.fams:
+        per += 3 * native.NCOND[f.closure] + geo + (9 if case.cfg.coupled else 0)
+    return 8.0 * per / len(case.fams)
+
+
+WORKLOADS = {
+    "jets2d": lambda mode: cases.case_crossing_jets(n=1024, steps=500, mode=mode),
+    "riemann1d": lambda mode: cases.case_riemann_1d(n=2000, steps=1000, mode=mode),
+}
+DESCR = {
+    "jets2d": "2-D crossing jets, 1024x1024 cells, AG closure, HLLE, MUSCL van Leer + Jameson sensor, "
+              "drag Schiller-Naumann + Ranz-Marshall heat, SSP-RK3, FP64",
+    "riemann1d": "1-D IG Riemann problem, 2000 cells, MUSCL minmod, Rusanov, drag, SSP-RK3, FP64",
+}
+
+
+def cpu_baseline(case, sample_steps: int):
+    """The oracle (OpenMP C++ restatement) on `sample_steps` steps of the same case."""
+    lib = native.bind(C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so")), "orc_")
+    r = cases.Runner(lib, case, "orc_")
+    t0 = time.perf_counter()
+    r.step(sample_steps)
+    dt = time.perf_counter() - t0
+    r.close()
+    return case.cell_stages(sample_steps) / dt / 1e6, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="jets2d", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="plain", choices=["plain", "batched"])
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        sys.exit(2)
+
+    mode = native.MODE_BATCHED if args.mode == "batched" else native.MODE_PLAIN
+    case = WORKLOADS[args.workload](mode)
+    gpu = native.load()
+    run = cases.Runner(gpu, case, "ice_gpu_")
+    if args.warmup:
+        run.step(args.warmup)
+    gpu.ice_gpu_set_kernel_timing(run.ctx, 1)
+    t0 = time.perf_counter()
+    st = run.step(args.steps)
+    wall = time.perf_counter() - t0
+    step_ms, kern_ms, launches = C.c_double(), C.c_double(), C.c_int64()
+    gpu.ice_gpu_timing(run.ctx, C.byref(step_ms), C.byref(kern_ms), C.byref(launches))
+    cell_stages = case.cell_stages(args.steps)
+    value = cell_stages / wall / 1e6
+    avg_kernel_s = kern_ms.value / max(launches.value, 1) / 1e3
+    cells_per_launch = cell_stages / max(launches.value, 1)
+    bytes_per_launch = cells_per_launch * stage_kernel_bytes(case)
+    achieved = bytes_per_launch / avg_kernel_s / 1e9 if avg_kernel_s > 0 else None
+
+    cpu = None
+    if args.cpu_steps > 0:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        sample = WORKLOADS[args.workload](native.MODE_PLAIN)
+        v, secs = cpu_baseline(sample, args.cpu_steps)
+        cpu = {"value": v, "unit": "Mcell-stage/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same workload from the same initial state "
+                         f"({secs:.1f} s on {threads} OpenMP threads)"}
+
+    out = {
+        "metric": "Mcell-stage updates/s (plain vs batched+surrogate, 1 MI355X) at fixed L2 err",
+        "value": value,
+        "unit": "Mcell-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": DESCR[args.workload], "mode": args.mode, "cells": case.cells,
+                   "families": case.cfg.ngroups, "rk_stages": case.cfg.nrk,
+                   "event_ms_per_step": step_ms.value / args.steps, "iter": st.iter},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "kernel": "k_stage (fused interior flux + source + residual + RK update)",
+                     "kernel_avg_ms": avg_kernel_s * 1e3, "bytes_per_cell": stage_kernel_bytes(case),
+                     "launches": launches.value},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

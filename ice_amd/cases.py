@@ -92,49 +92,74 @@ def make_family(closure: int, riemann: Optional[int] = None, rho_mat=2700.0, cs=
 
 def face_cells(dims):
     """(f, i, j, k) of every face cell of a block, faces 1..6, first tangential index fastest."""
+    return [tuple(int(x) for x in row) for row in face_cell_array(dims)]
+
+
+def face_cell_array(dims) -> np.ndarray:
+    """(n, 4) int array of (f, i, j, k) of every face cell, in face_cells order."""
     ni, nj, nk = dims
     out = []
     for f in range(1, 7):
         if f <= 2:
-            i = 1 if f == 1 else ni
-            out += [(f, i, j, k) for k in range(1, nk + 1) for j in range(1, nj + 1)]
+            J, K = np.meshgrid(np.arange(1, nj + 1), np.arange(1, nk + 1), indexing="xy")
+            I = np.full_like(J, 1 if f == 1 else ni)
         elif f <= 4:
-            j = 1 if f == 3 else nj
-            out += [(f, i, j, k) for k in range(1, nk + 1) for i in range(1, ni + 1)]
+            I, K = np.meshgrid(np.arange(1, ni + 1), np.arange(1, nk + 1), indexing="xy")
+            J = np.full_like(I, 1 if f == 3 else nj)
         else:
-            k = 1 if f == 5 else nk
-            out += [(f, i, j, k) for j in range(1, nj + 1) for i in range(1, ni + 1)]
-    return out
+            I, J = np.meshgrid(np.arange(1, ni + 1), np.arange(1, nj + 1), indexing="xy")
+            K = np.full_like(I, 1 if f == 5 else nk)
+        out.append(np.stack([np.full(I.size, f), I.ravel(), J.ravel(), K.ravel()], axis=1))
+    return np.concatenate(out).astype(np.int32)
 
 
 class TableBuilder:
-    """Accumulates the boundary table of a case, block by block."""
+    """Accumulates the boundary table of a case, block by block: one row per face
+    cell, fanned out over the families (one copy for all families)."""
 
     def __init__(self, ngroups: int):
         self.ng = ngroups
-        self.cells = []     # per face cell: (b, i, j, k, f, type, extra ints, extra reals)
+        self.ints = []      # (n, REC_NI) arrays, family column filled later
+        self.reals = []
         self.donor = []
         self.weight = []
 
+    def add_block(self, b: int, dims, code_fn, ints_fn=None, reals_fn=None):
+        """code_fn(f, i, j, k arrays) -> codes; ints_fn/reals_fn(rows, fijk) fill payload columns in place."""
+        fijk = face_cell_array(dims)
+        n = fijk.shape[0]
+        ri = np.zeros((n, REC_NI), np.int32)
+        rd = np.zeros((n, REC_ND), np.float64)
+        ri[:, R_B] = b
+        ri[:, R_F], ri[:, R_I], ri[:, R_J], ri[:, R_K] = fijk[:, 0], fijk[:, 1], fijk[:, 2], fijk[:, 3]
+        ri[:, R_TYPE] = code_fn(fijk[:, 0], fijk[:, 1], fijk[:, 2], fijk[:, 3])
+        if ints_fn is not None:
+            ints_fn(ri, fijk)
+        if reals_fn is not None:
+            reals_fn(rd, fijk)
+        self.ints.append(ri)
+        self.reals.append(rd)
+
     def add(self, b, f, i, j, k, code, ints=None, reals=None):
-        self.cells.append((b, i, j, k, f, code, dict(ints or {}), dict(reals or {})))
+        ri = np.zeros((1, REC_NI), np.int32)
+        rd = np.zeros((1, REC_ND), np.float64)
+        ri[0, R_B], ri[0, R_I], ri[0, R_J], ri[0, R_K], ri[0, R_F], ri[0, R_TYPE] = b, i, j, k, f, code
+        for key, v in (ints or {}).items():
+            ri[0, key] = v
+        for key, v in (reals or {}).items():
+            rd[0, key] = v
+        self.ints.append(ri)
+        self.reals.append(rd)
 
     def arrays(self):
-        n = len(self.cells) * self.ng
-        reci = np.zeros((n, REC_NI), np.int32)
-        recd = np.zeros((n, REC_ND), np.float64)
-        for c, (b, i, j, k, f, code, ints, reals) in enumerate(self.cells):
-            for p in range(1, self.ng + 1):
-                r = c * self.ng + (p - 1)
-                reci[r, R_B], reci[r, R_I], reci[r, R_J], reci[r, R_K] = b, i, j, k
-                reci[r, R_F], reci[r, R_TYPE], reci[r, R_P] = f, code, p
-                for key, v in ints.items():
-                    reci[r, key] = v
-                for key, v in reals.items():
-                    recd[r, key] = v
+        ri = np.concatenate(self.ints)
+        rd = np.concatenate(self.reals)
+        reci = np.repeat(ri, self.ng, axis=0)
+        recd = np.repeat(rd, self.ng, axis=0)
+        reci[:, R_P] = np.tile(np.arange(1, self.ng + 1, dtype=np.int32), ri.shape[0])
         donor = np.asarray(self.donor, np.int32).reshape(-1, 4) if self.donor else np.zeros((0, 4), np.int32)
         weight = np.asarray(self.weight, np.float64)
-        return reci, recd, donor, weight
+        return np.ascontiguousarray(reci), np.ascontiguousarray(recd), donor, weight
 
 
 def padded_state(dims, nc, fill_fn) -> np.ndarray:
@@ -184,8 +209,7 @@ def case_riemann_1d(n: int = 2000, steps: int = 1000, limiter: int = 1, drag: bo
     g5[4] = 300.0
     gas = gas_arrays(g5, 287.05, 1.4, 0.026, 1.8e-5) if drag else None
     tb = TableBuilder(1)
-    for (f, i, j, k) in face_cells(dims):
-        tb.add(1, f, i, j, k, 400 if f <= 2 else 0)
+    tb.add_block(1, dims, lambda f, i, j, k: np.where(f <= 2, 400, 0))
     reci, recd, donor, weight = tb.arrays()
     cfg = make_config(ngroups=1, nrk=3, time_accurate=True, ndir=1, limiter=limiter, coupled=drag,
                       drag_select=4, heat_select=8, cfl=0.5, mode=mode)
@@ -250,19 +274,25 @@ def case_crossing_jets(n: int = 1024, steps: int = 500, seed: int = 20261204, mo
     prim = padded_state(dims, 12, fill)
     tb = TableBuilder(1)
     g = rho_j * spd * math.cos(math.radians(30.0))
-    for (f, i, j, k) in face_cells(dims):
-        if f == 1:
-            y = (j - 0.5) * d
-            if 0.25 <= y <= 0.35 or 0.65 <= y <= 0.75:
-                ang = math.radians(30.0) if y < 0.5 else -math.radians(30.0)
-                tb.add(1, f, i, j, k, 402, reals={RD_MASSFLUX: g, RD_VELOCITY: spd, RD_ALPHA: ang, RD_BETA: 0.0,
-                                                  RD_TEMPERATURE: 300.0, RD_RADIUS: radius})
-            else:
-                tb.add(1, f, i, j, k, 300)
-        elif f <= 4:
-            tb.add(1, f, i, j, k, 400)
-        else:
-            tb.add(1, f, i, j, k, 0)
+
+    def jet(f, j):
+        y = (j - 0.5) * d
+        return (f == 1) & (((y >= 0.25) & (y <= 0.35)) | ((y >= 0.65) & (y <= 0.75)))
+
+    def codes(f, i, j, k):
+        return np.where(jet(f, j), 402, np.where(f == 1, 300, np.where(f <= 4, 400, 0)))
+
+    def reals(rd, fijk):
+        m = jet(fijk[:, 0], fijk[:, 2])
+        y = (fijk[:, 2] - 0.5) * d
+        rd[m, RD_MASSFLUX] = g
+        rd[m, RD_VELOCITY] = spd
+        rd[m, RD_ALPHA] = np.where(y[m] < 0.5, math.radians(30.0), -math.radians(30.0))
+        rd[m, RD_BETA] = 0.0
+        rd[m, RD_TEMPERATURE] = 300.0
+        rd[m, RD_RADIUS] = radius
+
+    tb.add_block(1, dims, codes, reals_fn=reals)
     reci, recd, donor, weight = tb.arrays()
     cfg = make_config(ngroups=1, nrk=3, time_accurate=time_accurate, ndir=2, limiter=3, shock_detector=True,
                       coupled=True, drag_select=4, heat_select=6, cfl=0.5, mode=mode)

@@ -512,3 +512,205 @@ ICE_DEV double rk_weight(int nrk, int irk) {
 }
 
 }  // namespace ice
+
+namespace ice {
+
+// ------------------------------------------------------------- streaming forms
+// Component v of the conservative vector and of the physical flux, evaluated
+// with exactly the operations of prim2cons / physflux, so a Riemann solver can
+// emit one flux component at a time without holding the four vectors.
+template <int M> ICE_DEV double cons_v(int v, const double* q, const MatDev& m) {
+  if (v == 0) return q[0];
+  if (v <= 3) return q[0] * q[v];
+  if constexpr (M == 0) {
+    if (v == 4) {
+      double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+      return q[0] * m.cs * q[4] + 0.5 * q[0] * n2;
+    }
+    return q[5];
+  } else if constexpr (M == 1) {
+    double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    double c4 = q[0] * n2 + 3.0 * q[4];
+    if (v == 4) return c4;
+    if (v == 5) return q[0] * m.cs * q[5] + 0.5 * c4;
+    return q[6];
+  } else {
+    switch (v) {
+      case 4: return q[0] * q[1] * q[1] + q[4];
+      case 5: return q[0] * q[1] * q[2] + q[5];
+      case 6: return q[0] * q[1] * q[3] + q[6];
+      case 7: return q[0] * q[2] * q[2] + q[7];
+      case 8: return q[0] * q[2] * q[3] + q[8];
+      case 9: return q[0] * q[3] * q[3] + q[9];
+      case 10: {
+        double c4 = q[0] * q[1] * q[1] + q[4], c7 = q[0] * q[2] * q[2] + q[7], c9 = q[0] * q[3] * q[3] + q[9];
+        return q[0] * m.cs * q[10] + 0.5 * (c4 + c7 + c9);
+      }
+      default: return q[11];
+    }
+  }
+}
+
+// scalars of the physical flux of a state across a face of normal n
+struct FluxScal {
+  double n2, un, f0, p1n, p2n, p3n;
+};
+template <int M> ICE_DEV FluxScal flux_scal(const double* q, const double* n) {
+  FluxScal s;
+  s.n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  s.un = q[1] * n[0] + q[2] * n[1] + q[3] * n[2];
+  s.f0 = q[0] * s.un;
+  if constexpr (M == 2) {
+    s.p1n = q[4] * n[0] + q[5] * n[1] + q[6] * n[2];
+    s.p2n = q[5] * n[0] + q[7] * n[1] + q[8] * n[2];
+    s.p3n = q[6] * n[0] + q[8] * n[1] + q[9] * n[2];
+  } else {
+    s.p1n = s.p2n = s.p3n = 0.0;
+  }
+  return s;
+}
+template <int M>
+ICE_DEV double flux_v(int v, const double* q, const double* n, const FluxScal& s, const MatDev& m) {
+  if (v == 0) return s.f0;
+  if constexpr (M == 0) {
+    if (v <= 3) return s.f0 * q[v];
+    if (v == 4) return s.f0 * (0.5 * s.n2 + m.cs * q[4]);
+    return q[5] * s.un;
+  } else if constexpr (M == 1) {
+    if (v <= 3) return s.f0 * q[v] + q[4] * n[v - 1];
+    if (v == 4) return s.f0 * s.n2 + 5.0 * q[4] * s.un;
+    if (v == 5) return s.f0 * (0.5 * s.n2 + m.cs * q[5]) + 2.5 * q[4] * s.un;
+    return q[6] * s.un;
+  } else {
+    switch (v) {
+      case 1: return s.f0 * q[1] + s.p1n;
+      case 2: return s.f0 * q[2] + s.p2n;
+      case 3: return s.f0 * q[3] + s.p3n;
+      case 4: return s.f0 * q[1] * q[1] + s.un * q[4] + q[1] * s.p1n + q[1] * s.p1n;
+      case 5: return s.f0 * q[1] * q[2] + s.un * q[5] + q[1] * s.p2n + q[2] * s.p1n;
+      case 6: return s.f0 * q[1] * q[3] + s.un * q[6] + q[1] * s.p3n + q[3] * s.p1n;
+      case 7: return s.f0 * q[2] * q[2] + s.un * q[7] + q[2] * s.p2n + q[2] * s.p2n;
+      case 8: return s.f0 * q[2] * q[3] + s.un * q[8] + q[2] * s.p3n + q[3] * s.p2n;
+      case 9: return s.f0 * q[3] * q[3] + s.un * q[9] + q[3] * s.p3n + q[3] * s.p3n;
+      case 10:
+        return s.f0 * (0.5 * s.n2 + 0.5 * (q[4] + q[7] + q[9]) / (q[0] + kEps) + m.cs * q[10]) + q[1] * s.p1n +
+               q[2] * s.p2n + q[3] * s.p3n;
+      default: return q[11] * s.un;
+    }
+  }
+}
+
+// The Riemann flux of states ql | qr times the face area, each component added
+// to (MODE 0) or subtracted from (MODE 1) the accumulator r, or stored in r (MODE 2).
+template <int M, int S, int MODE>
+ICE_DEV void riemann_acc(const double* q1, const double* q4, const double* n, double area, double* r,
+                         const MatDev& m) {
+  constexpr int NC = Closure<M>::NC;
+  if constexpr (S == 0) {
+    double f[NC];
+    riemann<M, 0>(q1, q4, n, f, m);
+#pragma unroll
+    for (int v = 0; v < NC; ++v) r[v] = MODE == 2 ? f[v] * area : (MODE == 1 ? r[v] - f[v] * area : r[v] + f[v] * area);
+  } else {
+    double vn1 = q1[1] * n[0] + q1[2] * n[1] + q1[3] * n[2];
+    double vn4 = q4[1] * n[0] + q4[2] * n[1] + q4[3] * n[2];
+    double a1 = wavespeed<M>(q1, n), a4 = wavespeed<M>(q4, n);
+    FluxScal s1 = flux_scal<M>(q1, n), s4 = flux_scal<M>(q4, n);
+    if constexpr (S == 1) {
+      double A = dmax2(dmax2(dmax2(fabs(vn1 - a1), fabs(vn4 - a4)), fabs(vn1 + a1)), fabs(vn4 + a4));
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        double fv = 0.5 * (flux_v<M>(v, q1, n, s1, m) + flux_v<M>(v, q4, n, s4, m) -
+                           A * (cons_v<M>(v, q4, m) - cons_v<M>(v, q1, m)));
+        r[v] = MODE == 2 ? fv * area : (MODE == 1 ? r[v] - fv * area : r[v] + fv * area);
+      }
+    } else {
+      double sq1 = sqrt(q1[0]), sq4 = sqrt(q4[0]);
+      double uR = (sq1 * q1[1] + sq4 * q4[1]) / (sq1 + sq4);
+      double vR = (sq1 * q1[2] + sq4 * q4[2]) / (sq1 + sq4);
+      double wR = (sq1 * q1[3] + sq4 * q4[3]) / (sq1 + sq4);
+      double vnR = uR * n[0] + vR * n[1] + wR * n[2];
+      double aR = (sq1 * a1 + sq4 * a4) / (sq1 + sq4);
+      double S1 = dmin2(dmin2(0.0, vn1 - a1), vnR - aR);
+      double S4 = dmax2(dmax2(0.0, vn4 + a4), vnR + aR);
+      double k0 = -S1, k1 = S1 * S4, k2 = S4;
+      int loc = 0;
+      double mn = k0;
+      if (k1 < mn) { loc = 1; mn = k1; }
+      if (k2 < mn) loc = 2;
+      if (loc != 1) {
+        const double* qu = loc == 0 ? q1 : q4;
+        FluxScal su = loc == 0 ? s1 : s4;
+#pragma unroll
+        for (int v = 0; v < NC; ++v) {
+          double fv = flux_v<M>(v, qu, n, su, m);
+          r[v] = MODE == 2 ? fv * area : (MODE == 1 ? r[v] - fv * area : r[v] + fv * area);
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NC; ++v) {
+          double fv = (S4 * flux_v<M>(v, q1, n, s1, m) - S1 * flux_v<M>(v, q4, n, s4, m) +
+                       S1 * S4 * (cons_v<M>(v, q4, m) - cons_v<M>(v, q1, m))) /
+                      (S4 - S1);
+          r[v] = MODE == 2 ? fv * area : (MODE == 1 ? r[v] - fv * area : r[v] + fv * area);
+        }
+      }
+    }
+  }
+}
+
+// MUSCL reconstruction reading the stencil slot by slot from memory: p points
+// at slot 0 of the cell before the face's lower cell, consecutive stencil
+// cells are `stride` apart, slots N apart.
+template <int M, int LIM = -1>
+ICE_DEV bool reconstruct_mem(int lim, const double* __restrict__ p, long long stride, long long N, double dl0,
+                             double dl1, double dl2, double dll, double dlr, double beta, double* ql, double* qr) {
+  if constexpr (LIM >= 0) lim = LIM;
+  constexpr int NC = Closure<M>::NC;
+  double cxl, cyl, cxr, cyr;
+  if (dll > 0.0) { cxl = dl1 / dll; cyl = dl0 / dll; } else { cxl = 2.0; cyl = 2.0; }
+  if (dlr > 0.0) { cxr = dl2 / dlr; cyr = dl1 / dlr; } else { cxr = 2.0; cyr = 2.0; }
+  double r0 = p[0], r1 = p[stride], r2 = p[2 * stride], r3 = p[3 * stride];
+  bool first_l = r0 <= kRhoEmpty || r1 <= kRhoEmpty || r2 <= kRhoEmpty;
+  bool first_r = r1 <= kRhoEmpty || r2 <= kRhoEmpty || r3 <= kRhoEmpty;
+  if (first_l) {
+#pragma unroll
+    for (int v = 0; v < NC; ++v) ql[v] = p[v * N + stride];
+  }
+  if (first_r) {
+#pragma unroll
+    for (int v = 0; v < NC; ++v) qr[v] = p[v * N + 2 * stride];
+  }
+  bool ok_l = first_l, ok_r = first_r;
+  double limval = beta;
+  while (!ok_l || !ok_r) {
+#pragma unroll
+    for (int v = 0; v < NC; ++v) {
+      const double* pv = p + v * N;
+      double prev = pv[0], local = pv[stride], next = pv[2 * stride], next2 = pv[3 * stride];
+      double s1 = (next - local) / dl1;
+      if (!first_l) {
+        double s0 = (local - prev) / dl0;
+        ql[v] = local + limiter(lim, s1, s0, cxl, cyl) * limval * dll;
+      }
+      if (!first_r) {
+        double s2 = (next2 - next) / dl2;
+        qr[v] = next - limiter(lim, s2, s1, cxr, cyr) * limval * dlr;
+      }
+    }
+    if (!first_l) ok_l = checkprim<M>(ql);
+    if (!first_r) ok_r = checkprim<M>(qr);
+    limval = 0.5 * limval;
+    if (limval < 1.e-18 && !(ok_l && ok_r)) {
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        ql[v] = p[v * N + stride];
+        qr[v] = p[v * N + 2 * stride];
+      }
+      return checkprim<M>(ql) && checkprim<M>(qr);
+    }
+  }
+  return true;
+}
+
+}  // namespace ice

@@ -14,7 +14,7 @@
 
 #include "../../include/ice_gpu.h"
 #include "batched.hpp"
-#include "stage_kernels.hpp"
+#include "launch.hpp"
 
 using namespace ice;
 
@@ -67,6 +67,7 @@ struct ice_gpu_ctx {
   long long kernel_launches = 0;
   std::vector<hipEvent_t> kev;
   BatchedPlan* batched = nullptr;
+  int tile[3] = {-1, -1, -1};  // cells per stage tile; -1 default shape, 0 per-cell kernel
   std::string err;
 };
 
@@ -208,8 +209,13 @@ int build_lists(ice_gpu_ctx* c) {
     FamDev& F = c->fam[p - 1];
     std::vector<int> grp, own, bfirst, brec;
     std::vector<long long> bg;
-    for (int n = 1; n <= nrec; ++n)
-      if (c->rec[n - 1].v[ICE_R_P] == p) grp.push_back(n - 1);
+    for (int n = 1; n <= nrec; ++n) {
+      const int* v = c->rec[n - 1].v;
+      if (v[ICE_R_P] != p) continue;
+      // a planar record of a flat direction fills ghosts nothing reads (BlkDev::flat)
+      if (v[ICE_R_TYPE] == 0 && (c->blk[v[ICE_R_B] - 1].flat >> ((v[ICE_R_F] - 1) / 2)) & 1) continue;
+      grp.push_back(n - 1);
+    }
     for (int n0 : grp) {
       const HRec& R = c->rec[n0];
       if ((R.v[ICE_R_TYPE] == 101 || R.v[ICE_R_TYPE] == 201) && R.owner) own.push_back(n0);
@@ -291,84 +297,67 @@ FamArgs fam_args(ice_gpu_ctx* c, int p) {
 }
 
 // ---------------------------------------------------------------- dispatch
-template <int M> void launch_dt(ice_gpu_ctx* c, const FamArgs& a) {
-  for (size_t b = 0; b < c->blk.size(); ++b)
-    hipLaunchKernelGGL(k_dt<M>, dim3(grid_of(c->blk[b].cells)), dim3(kThreads), 0, c->stream, a, (int)b);
+__global__ void k_step_begin(DevStatus* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) st->dtbits = (unsigned long long)__double_as_longlong(1e+5);
 }
 
-template <int M, int S, bool SD, bool CP>
-void launch_stage_kernels(ice_gpu_ctx* c, FamArgs& a) {
-  constexpr int NC = Closure<M>::NC;
-  const FamDev& F = c->fam[a.fam];
-  int nb = (int)c->blk.size();
-  // ghost fill: first then second layer, records of the family in table order
-  a.list = F.grp;
-  a.nlist = F.ngrp;
-  if (F.ngrp) {
-    hipLaunchKernelGGL(k_ghost1<M>, dim3(grid_of(F.ngrp)), dim3(kThreads), 0, c->stream, a);
-    hipLaunchKernelGGL(k_ghost2<M>, dim3(grid_of(F.ngrp)), dim3(kThreads), 0, c->stream, a);
+__global__ void k_dt_end(DevStatus* st, int time_accurate) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double dt = __longlong_as_double((long long)st->dtbits);
+    st->dtglobal = dt;
+    if (time_accurate) st->time = st->time + dt;
   }
-  if (SD)
-    for (int b = 0; b < nb; ++b)
-      hipLaunchKernelGGL(k_beta, dim3(grid_of(c->blk[b].cells)), dim3(kThreads), 0, c->stream, a, b);
-  if (F.nown) {
-    a.list = F.own;
-    a.nlist = F.nown;
-    hipLaunchKernelGGL((k_conn<M, S, SD>), dim3(grid_of(F.nown)), dim3(kThreads), 0, c->stream, a);
+}
+
+void launch_dt(ice_gpu_ctx* c, int p, const FamArgs& a) {
+  int cl = c->fams[p].closure, nb = (int)c->blk.size();
+  if (cl == ICE_MK) launch_dt_mk(c->stream, c->blk.data(), nb, a);
+  else if (cl == ICE_IG) launch_dt_ig(c->stream, c->blk.data(), nb, a);
+  else launch_dt_ag(c->stream, c->blk.data(), nb, a);
+}
+
+// The stage tile of a block: the context's shape, else one whose face grids in
+// every resolved direction fit the 256 threads of a workgroup.
+Tile stage_tile(ice_gpu_ctx* c, const BlkDev& B) {
+  Tile T;
+  if (c->tile[0] == 0) {
+    T.tx = T.ty = T.tz = 0;
+    T.ntx = T.nty = T.ntz = 0;
+    return T;
   }
-  if (F.nbcell) hipLaunchKernelGGL((k_bcell<M, S, SD>), dim3(grid_of(F.nbcell)), dim3(kThreads), 0, c->stream, a);
-  if (!c->cfg.irs) {
-    for (int b = 0; b < nb; ++b) {
-      bool timed = c->kernel_timing;
-      if (timed) {
-        hipEvent_t e0, e1;
-        (void)hipEventCreate(&e0);
-        (void)hipEventCreate(&e1);
-        (void)hipEventRecord(e0, c->stream);
-        hipLaunchKernelGGL((k_stage<M, S, SD, CP, false>), dim3(grid_of(c->blk[b].cells)), dim3(kThreads), 0,
-                           c->stream, a, b);
-        (void)hipEventRecord(e1, c->stream);
-        c->kev.push_back(e0);
-        c->kev.push_back(e1);
-      } else {
-        hipLaunchKernelGGL((k_stage<M, S, SD, CP, false>), dim3(grid_of(c->blk[b].cells)), dim3(kThreads), 0,
-                           c->stream, a, b);
-      }
-    }
+  int tx, ty, tz;
+  if (c->tile[0] > 0) {
+    tx = c->tile[0]; ty = c->tile[1]; tz = c->tile[2];
+  } else if (B.nj == 1 && B.nk == 1) {
+    tx = 255; ty = 1; tz = 1;
+  } else if (B.nk == 1) {
+    tx = 15; ty = 16; tz = 1;
   } else {
-    for (int b = 0; b < nb; ++b)
-      hipLaunchKernelGGL((k_stage<M, S, SD, CP, true>), dim3(grid_of(c->blk[b].cells)), dim3(kThreads), 0, c->stream,
-                         a, b);
-    for (int d = 0; d < c->cfg.ndir; ++d)
-      for (int sw = 1; sw <= 2; ++sw)
-        for (int b = 0; b < nb; ++b)
-          hipLaunchKernelGGL(k_irs<NC>, dim3(grid_of(c->blk[b].cells)), dim3(kThreads), 0, c->stream, a, b, d, sw);
-    for (int b = 0; b < nb; ++b)
-      hipLaunchKernelGGL(k_update<M>, dim3(grid_of(c->blk[b].cells)), dim3(kThreads), 0, c->stream, a, b);
+    tx = 7; ty = 6; tz = 5;
   }
-}
-
-template <int M, int S> void dispatch_sd_cp(ice_gpu_ctx* c, FamArgs& a) {
-  bool sd = c->cfg.shock_detector, cp = c->cfg.coupled;
-  if (sd && cp) launch_stage_kernels<M, S, true, true>(c, a);
-  else if (sd) launch_stage_kernels<M, S, true, false>(c, a);
-  else if (cp) launch_stage_kernels<M, S, false, true>(c, a);
-  else launch_stage_kernels<M, S, false, false>(c, a);
+  tx = std::min(tx, B.ni); ty = std::min(ty, B.nj); tz = std::min(tz, B.nk);
+  T.tx = tx; T.ty = ty; T.tz = tz;
+  T.ntx = (B.ni + tx - 1) / tx; T.nty = (B.nj + ty - 1) / ty; T.ntz = (B.nk + tz - 1) / tz;
+  return T;
 }
 
 void dispatch_stage(ice_gpu_ctx* c, FamArgs& a) {
-  int cl = c->fams[a.fam].closure, rs = c->fams[a.fam].riemann;
-  if (cl == ICE_MK) {
-    if (rs == ICE_SAUREL) dispatch_sd_cp<0, 0>(c, a);
-    else if (rs == ICE_RUSANOV) dispatch_sd_cp<0, 1>(c, a);
-    else dispatch_sd_cp<0, 2>(c, a);
-  } else if (cl == ICE_IG) {
-    if (rs == ICE_HLLE) dispatch_sd_cp<1, 2>(c, a);
-    else dispatch_sd_cp<1, 1>(c, a);
-  } else {
-    if (rs == ICE_HLLE) dispatch_sd_cp<2, 2>(c, a);
-    else dispatch_sd_cp<2, 1>(c, a);
-  }
+  const FamDev& F = c->fam[a.fam];
+  std::vector<Tile> tiles(c->blk.size());
+  for (size_t b = 0; b < c->blk.size(); ++b) tiles[b] = stage_tile(c, c->blk[b]);
+  StageLaunch L;
+  L.stream = c->stream;
+  L.blk = c->blk.data();
+  L.tiles = tiles.data();
+  L.nb = (int)c->blk.size();
+  L.grp = F.grp; L.ngrp = F.ngrp; L.own = F.own; L.nown = F.nown; L.nbcell = F.nbcell;
+  L.riemann = c->fams[a.fam].riemann; L.sd = c->cfg.shock_detector; L.coupled = c->cfg.coupled;
+  L.irs = c->cfg.irs; L.limiter = c->cfg.limiter; L.ndir = c->cfg.ndir;
+  L.timing = c->kernel_timing; L.kev = &c->kev;
+  int cl = c->fams[a.fam].closure;
+  if (cl == ICE_MK) launch_stage_mk(L, a);
+  else if (cl == ICE_IG) launch_stage_ig(L, a);
+  else launch_stage_ag(L, a);
 }
 
 // the roles of the three buffers at stage srk: read cur (and old), write nxt
@@ -388,10 +377,7 @@ int plain_step(ice_gpu_ctx* c) {
     FamDev& F = c->fam[p];
     F.old = F.cur;  // state_copy: the step's start state stays in its buffer
     FamArgs a = fam_args(c, p);
-    int cl = c->fams[p].closure;
-    if (cl == ICE_MK) launch_dt<0>(c, a);
-    else if (cl == ICE_IG) launch_dt<1>(c, a);
-    else launch_dt<2>(c, a);
+    launch_dt(c, p, a);
   }
   hipLaunchKernelGGL(k_dt_end, dim3(1), dim3(64), 0, c->stream, c->d_st, c->cfg.time_accurate);
   for (int p = 0; p < ng; ++p) {
@@ -453,6 +439,7 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
   long long off = 0;
   for (int b = 0; b < nb; ++b) {
     BlkDev B;
+    std::memset(&B, 0, sizeof B);
     B.ni = dims[3 * b]; B.nj = dims[3 * b + 1]; B.nk = dims[3 * b + 2];
     if (B.ni < 1 || B.nj < 1 || B.nk < 1) return fail(c, "[ICE::create] block %d has an empty dimension", b + 1);
     B.NI = B.ni + 4; B.NJ = B.nj + 4; B.NK = B.nk + 4;
@@ -617,6 +604,16 @@ int ice_gpu_set_boundaries(ice_gpu_ctx* c, int nrec, const int32_t* reci, const 
     HIPCHK(hipMemcpy(c->d_donor, donor, (size_t)ndonor * 4 * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_weight, weight, (size_t)ndonor * sizeof(double), hipMemcpyHostToDevice));
   }
+  // directions of one cell whose faces are all planar (0), in every family
+  for (size_t b = 0; b < c->blk.size(); ++b) {
+    BlkDev& B = c->blk[b];
+    const int n[3] = {B.ni, B.nj, B.nk};
+    B.flat = 0;
+    for (int d = 0; d < 3; ++d) B.flat |= (n[d] == 1) << d;
+  }
+  for (const HRec& H : c->rec)
+    if (H.v[ICE_R_TYPE] != 0) c->blk[H.v[ICE_R_B] - 1].flat &= ~(1 << ((H.v[ICE_R_F] - 1) / 2));
+  HIPCHK(hipMemcpy(c->d_blk, c->blk.data(), c->blk.size() * sizeof(BlkDev), hipMemcpyHostToDevice));
   if (build_lists(c)) return 1;
   return 0;
 }
@@ -787,6 +784,28 @@ int ice_gpu_timing(ice_gpu_ctx* c, double* step_ms, double* kernel_ms, int64_t* 
   if (step_ms) *step_ms = c->step_ms;
   if (kernel_ms) *kernel_ms = c->kernel_ms;
   if (launches) *launches = c->kernel_launches;
+  return 0;
+}
+
+int ice_gpu_set_tiles(ice_gpu_ctx* c, int tx, int ty, int tz) {
+  if (tx == 0) {
+    c->tile[0] = c->tile[1] = c->tile[2] = 0;
+    return 0;
+  }
+  if (tx < 0) {
+    c->tile[0] = c->tile[1] = c->tile[2] = -1;
+    return 0;
+  }
+  for (const BlkDev& B : c->blk) {
+    int n[3] = {B.ni, B.nj, B.nk}, t[3] = {std::min(tx, B.ni), std::min(ty, B.nj), std::min(tz, B.nk)};
+    bool bad = ty < 1 || tz < 1 || t[0] * t[1] * t[2] > kThreads;
+    for (int d = 0; d < 3; ++d)
+      if (n[d] > 1) bad = bad || t[0] * t[1] * t[2] / t[d] * (t[d] + 1) > kMaxTileFaces;
+    if (bad)
+      return fail(c, "[ICE::set_tiles] a %d x %d x %d tile has more cells or faces than the %d threads of a workgroup",
+                  tx, ty, tz, kThreads);
+  }
+  c->tile[0] = tx; c->tile[1] = ty; c->tile[2] = tz;
   return 0;
 }
 

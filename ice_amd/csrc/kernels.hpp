@@ -15,6 +15,11 @@ struct BlkDev {
   int ni, nj, nk, NI, NJ, NK;
   long long off;   // padded offset of the block
   long long cells; // ni*nj*nk
+  // bit d: direction d has one cell and only planar (0) faces, so its ghosts
+  // are copies of the cell, read by nothing but the sensor, whose term for the
+  // direction is then exactly 0 (|(r - 2r + r)/(4r + 1e-30)|)
+  int flat;
+  int pad;
 };
 
 // A boundary record (ICE_bc_type) with its pairing
@@ -96,35 +101,28 @@ template <int NC> ICE_DEV void store_cell(double* p, long long N, long long g, c
 
 // ------------------------------------------------------------ interior face
 // Face between padded cell g and g + stride (direction d), its lower cell at
-// index m (1-based) of n along d.  Mod_Fluxes.f90:599-685.
-template <int M, int S, bool SD>
-ICE_DEV bool face_flux(const FamArgs& a, long long g, long long stride, int d, int m, int n, double* fl) {
+// index m (1-based) of n along d (Mod_Fluxes.f90:599-685); its flux times the
+// area is added to (MODE 0), subtracted from (MODE 1) or stored in (MODE 2) r.
+template <int M, int S, bool SD, int MODE, int LIM = -1>
+ICE_DEV bool face_acc(const FamArgs& a, long long g, long long stride, int d, int m, int n, double* r) {
   constexpr int NC = Closure<M>::NC;
   const long long N = a.N;
-  double st[4][NC];
-  double len[4];
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    long long gs = g + (long long)(s - 1) * stride;
-    load_cell<NC>(a.cur, N, gs, st[s]);
-    len[s] = a.dl[(long long)d * N + gs];
-  }
+  const double* dl = a.dl + (long long)d * N;
+  double len0 = dl[g - stride], len1 = dl[g], len2 = dl[g + stride], len3 = dl[g + 2 * stride];
   double bf = SD ? dmin2(a.beta[g], a.beta[g + stride]) : 1.0;
-  if (m == 1 && a.inj[g - stride]) len[0] = 0.0;
-  if (m == n - 1 && a.inj[g + 2 * stride]) len[3] = 0.0;
+  if (m == 1 && a.inj[g - stride]) len0 = 0.0;
+  if (m == n - 1 && a.inj[g + 2 * stride]) len3 = 0.0;
   const double* fa = a.fa + (long long)(d * 4) * N;
   double nrm[3] = {fa[g], fa[N + g], fa[2 * N + g]};
   double area = fa[3 * N + g];
-  double dl0 = 0.5 * (len[0] + len[1]);
-  double dl1 = 0.5 * (len[1] + len[2]);
-  double dl2 = 0.5 * (len[2] + len[3]);
-  double dll = 0.5 * len[1];
-  double dlr = 0.5 * len[2];
+  double dl0 = 0.5 * (len0 + len1);
+  double dl1 = 0.5 * (len1 + len2);
+  double dl2 = 0.5 * (len2 + len3);
+  double dll = 0.5 * len1;
+  double dlr = 0.5 * len2;
   double ql[NC], qr[NC];
-  bool ok = reconstruct<M>(a.limiter, st[0], st[1], st[2], st[3], dl0, dl1, dl2, dll, dlr, bf, ql, qr);
-  riemann<M, S>(ql, qr, nrm, fl, a.mat);
-#pragma unroll
-  for (int v = 0; v < NC; ++v) fl[v] = fl[v] * area;
+  bool ok = reconstruct_mem<M, LIM>(a.limiter, a.cur + (g - stride), stride, N, dl0, dl1, dl2, dll, dlr, bf, ql, qr);
+  riemann_acc<M, S, MODE>(ql, qr, nrm, area, r, a.mat);
   return ok;
 }
 

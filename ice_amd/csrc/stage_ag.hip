@@ -1,0 +1,10 @@
+// Stage kernels of the anisotropic Gaussian (AG) closure: Rusanov or HLLE.
+#include "launch.hpp"
+
+namespace ice {
+void launch_stage_ag(const StageLaunch& L, const FamArgs& a) {
+  if (L.riemann == 2) launch_sd_cp<2, 2>(L, a);
+  else launch_sd_cp<2, 1>(L, a);
+}
+void launch_dt_ag(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<2>(s, blk, nb, a); }
+}  // namespace ice

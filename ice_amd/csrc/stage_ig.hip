@@ -1,0 +1,10 @@
+// Stage kernels of the isotropic Gaussian (IG) closure: Rusanov or HLLE.
+#include "launch.hpp"
+
+namespace ice {
+void launch_stage_ig(const StageLaunch& L, const FamArgs& a) {
+  if (L.riemann == 2) launch_sd_cp<1, 2>(L, a);
+  else launch_sd_cp<1, 1>(L, a);
+}
+void launch_dt_ig(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<1>(s, blk, nb, a); }
+}  // namespace ice

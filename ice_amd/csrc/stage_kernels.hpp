@@ -71,18 +71,6 @@ __global__ void __launch_bounds__(kThreads) k_dt(FamArgs a, int b) {
   block_min_atomic(dtc, &a.st->dtbits);
 }
 
-__global__ void k_step_begin(DevStatus* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) st->dtbits = (unsigned long long)__double_as_longlong(1e+5);
-}
-
-__global__ void k_dt_end(DevStatus* st, int time_accurate) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double dt = __longlong_as_double((long long)st->dtbits);
-    st->dtglobal = dt;
-    if (time_accurate) st->time = st->time + dt;
-  }
-}
-
 // ------------------------------------------------------------------ ghosts
 enum { kCopy = 0, kMirror = 1, kInject = 2 };
 
@@ -330,6 +318,7 @@ __global__ void __launch_bounds__(kThreads) k_ghost2(FamArgs a) {
 
 // ------------------------------------------------------------------ beta
 // tile_beta / SD_rho (Mod_Fluxes.f90:366-389)
+template <int M>
 __global__ void __launch_bounds__(kThreads) k_beta(FamArgs a, int b) {
   const BlkDev B = a.blk[b];
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -339,7 +328,12 @@ __global__ void __launch_bounds__(kThreads) k_beta(FamArgs a, int b) {
   long long g = gidx(B, i, j, k);
   long long sj = B.NI, sk = (long long)B.NI * B.NJ;
   const double* r = a.cur;
-  a.beta[g] = sd_weight(r[g], r[g - 1], r[g + 1], r[g - sj], r[g + sj], r[g - sk], r[g + sk]);
+  double rc = r[g];
+  // a flat direction's neighbours are the cell itself (see BlkDev::flat)
+  double ri0 = (B.flat & 1) ? rc : r[g - 1], ri1 = (B.flat & 1) ? rc : r[g + 1];
+  double rj0 = (B.flat & 2) ? rc : r[g - sj], rj1 = (B.flat & 2) ? rc : r[g + sj];
+  double rk0 = (B.flat & 4) ? rc : r[g - sk], rk1 = (B.flat & 4) ? rc : r[g + sk];
+  a.beta[g] = sd_weight(rc, ri0, ri1, rj0, rj1, rk0, rk1);
 }
 
 // ------------------------------------------------------------------ boundary fluxes
@@ -514,27 +508,12 @@ __global__ void __launch_bounds__(kThreads) k_stage(FamArgs a, int b) {
     int n = nd[d], m = idx[d];
     if (n < 2) continue;
     long long s = dir_stride(Bv, d);
-    double f[NC];
     if (m % 2 == 1) {
-      if (m < n) {
-        ok = face_flux<M, S, SD>(a, g, s, d, m, n, f) && ok;
-#pragma unroll
-        for (int v = 0; v < NC; ++v) r[v] = r[v] - f[v];
-      }
-      if (m > 1) {
-        ok = face_flux<M, S, SD>(a, g - s, s, d, m - 1, n, f) && ok;
-#pragma unroll
-        for (int v = 0; v < NC; ++v) r[v] = r[v] + f[v];
-      }
+      if (m < n) ok = face_acc<M, S, SD, 1>(a, g, s, d, m, n, r) && ok;
+      if (m > 1) ok = face_acc<M, S, SD, 0>(a, g - s, s, d, m - 1, n, r) && ok;
     } else {
-      ok = face_flux<M, S, SD>(a, g - s, s, d, m - 1, n, f) && ok;
-#pragma unroll
-      for (int v = 0; v < NC; ++v) r[v] = r[v] + f[v];
-      if (m < n) {
-        ok = face_flux<M, S, SD>(a, g, s, d, m, n, f) && ok;
-#pragma unroll
-        for (int v = 0; v < NC; ++v) r[v] = r[v] - f[v];
-      }
+      ok = face_acc<M, S, SD, 0>(a, g - s, s, d, m - 1, n, r) && ok;
+      if (m < n) ok = face_acc<M, S, SD, 1>(a, g, s, d, m, n, r) && ok;
     }
   }
   if (!ok) atomicOr(&a.st->bad_recon, 1);
@@ -604,6 +583,126 @@ __global__ void __launch_bounds__(kThreads) k_update(FamArgs a, int b) {
   double r[NC];
   load_cell<NC>(a.R, a.N, g, r);
   update_cell<M>(a, B, i, j, k, g, r);
+}
+
+}  // namespace ice
+
+namespace ice {
+
+// Tile of a block handled by one workgroup: TX x TY x TZ cells; the faces of
+// the tile in one direction are computed once each, by one thread, into LDS.
+struct Tile {
+  int tx, ty, tz;     // cells per tile
+  int ntx, nty, ntz;  // tiles per block
+};
+
+// slots of the face fluxes of one direction in LDS, structure of arrays
+constexpr int kMaxTileFaces = kThreads;
+
+// The stage with the interior faces shared through LDS (Mod_Fluxes.f90:421-521:
+// each face once, each cell its two faces per direction in the odd-then-even
+// order), then the cell's source, residual and update as in k_stage.
+template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
+__global__ void __launch_bounds__(kThreads) k_stage_tiled(FamArgs a, int b, Tile T) {
+  constexpr int NC = Closure<M>::NC;
+  __shared__ double fl[NC * kMaxTileFaces];
+  const BlkDev& Bref = a.blk[b];
+  const BlkDev B = Bref;
+  int tile = blockIdx.x;
+  int ox = (tile % T.ntx) * T.tx + 1;
+  int oy = ((tile / T.ntx) % T.nty) * T.ty + 1;
+  int oz = (tile / (T.ntx * T.nty)) * T.tz + 1;
+  int t = threadIdx.x;
+  int ncell = T.tx * T.ty * T.tz;
+  int ci = ox + t % T.tx, cj = oy + (t / T.tx) % T.ty, ck = oz + t / (T.tx * T.ty);
+  bool has_cell = t < ncell && ci <= B.ni && cj <= B.nj && ck <= B.nk;
+  long long g = has_cell ? gidx(B, ci, cj, ck) : 0;
+  double r[NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) r[v] = 0.0;
+  if (has_cell && (ci == 1 || ci == B.ni || cj == 1 || cj == B.nj || ck == 1 || ck == B.nk)) {
+    int c = a.bidx[g];
+    if (c >= 0) {
+#pragma unroll
+      for (int v = 0; v < NC; ++v) r[v] = a.bsum[(long long)c * NC + v];
+    }
+  }
+  bool ok = true;
+  const int nd[3] = {B.ni, B.nj, B.nk};
+  const int td[3] = {T.tx, T.ty, T.tz};
+  for (int d = 0; d < 3; ++d) {
+    int n = nd[d];
+    if (n < 2) continue;
+    // faces of the tile along d: lower-cell offset -1 .. t_d-1 along d
+    int fx = T.tx + (d == 0), fy = T.ty + (d == 1), fz = T.tz + (d == 2);
+    int nf = fx * fy * fz;
+    long long stride = dir_stride(B, d);
+    for (int f = t; f < nf; f += blockDim.x) {
+      int li = ox + f % fx - (d == 0);
+      int lj = oy + (f / fx) % fy - (d == 1);
+      int lk = oz + f / (fx * fy) - (d == 2);
+      int m = d == 0 ? li : (d == 1 ? lj : lk);
+      bool inside = m >= 1 && m <= n - 1 && li >= 1 && lj >= 1 && lk >= 1 && li <= B.ni && lj <= B.nj && lk <= B.nk;
+      if (inside) {
+        double q[NC];
+        ok = face_acc<M, S, SD, 2, LIM>(a, gidx(B, li, lj, lk), stride, d, m, n, q) && ok;
+#pragma unroll
+        for (int v = 0; v < NC; ++v) fl[v * kMaxTileFaces + f] = q[v];
+      }
+    }
+    __syncthreads();
+    if (has_cell) {
+      int ti = ci - ox, tj = cj - oy, tk = ck - oz;
+      // face index of the cell's lower face in this direction's face grid
+      int flo = ti + fx * (tj + fy * tk);
+      int fup = flo + (d == 0 ? 1 : (d == 1 ? fx : fx * fy));
+      int m = d == 0 ? ci : (d == 1 ? cj : ck);
+      if (m % 2 == 1) {
+        if (m < n) {
+#pragma unroll
+          for (int v = 0; v < NC; ++v) r[v] = r[v] - fl[v * kMaxTileFaces + fup];
+        }
+        if (m > 1) {
+#pragma unroll
+          for (int v = 0; v < NC; ++v) r[v] = r[v] + fl[v * kMaxTileFaces + flo];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NC; ++v) r[v] = r[v] + fl[v * kMaxTileFaces + flo];
+        if (m < n) {
+#pragma unroll
+          for (int v = 0; v < NC; ++v) r[v] = r[v] - fl[v * kMaxTileFaces + fup];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!ok) atomicOr(&a.st->bad_recon, 1);
+  if (!has_cell) return;
+  double src[NC];
+  if constexpr (COUPLED) {
+    double q[NC], tau;
+    load_cell<NC>(a.cur, a.N, g, q);
+    cell_source<M>(a, g, q, src, tau);
+    for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
+    if (a.last_stage) {
+      a.tau[g] = tau;
+      if (a.keep_source) store_cell<NC>(a.src, a.N, g, src);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < NC; ++v) src[v] = 0.0;
+  }
+  double vol = a.vol[g];
+  double dt = a.time_accurate ? a.st->dtglobal : a.dt[g];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) r[v] = (r[v] / vol + src[v]) * dt;
+  for (int f = 0; f < a.nfrozen; ++f) r[a.frozen[f] - 1] = 0.0;
+  if constexpr (WRITE_R) {
+    store_cell<NC>(a.R, a.N, g, r);
+  } else {
+    update_cell<M>(a, Bref, ci, cj, ck, g, r);
+  }
 }
 
 }  // namespace ice

@@ -1,0 +1,11 @@
+// Stage kernels of the monokinetic (MK) closure: Saurel, Rusanov or HLLE.
+#include "launch.hpp"
+
+namespace ice {
+void launch_stage_mk(const StageLaunch& L, const FamArgs& a) {
+  if (L.riemann == 0) launch_sd_cp<0, 0>(L, a);
+  else if (L.riemann == 1) launch_sd_cp<0, 1>(L, a);
+  else launch_sd_cp<0, 2>(L, a);
+}
+void launch_dt_mk(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<0>(s, blk, nb, a); }
+}  // namespace ice

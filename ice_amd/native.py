@@ -75,6 +75,7 @@ SIGNATURES = {
     "ice_gpu_residual_norm": [C.c_void_p, _DP, _DP],
     "ice_gpu_timing": [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "ice_gpu_set_kernel_timing": [C.c_void_p, C.c_int],
+    "ice_gpu_set_tiles": [C.c_void_p, C.c_int, C.c_int, C.c_int],
     "ice_gpu_last_error": [C.c_void_p, C.c_char_p, C.c_int],
     "ice_gpu_destroy": [C.c_void_p],
 }

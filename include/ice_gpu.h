@@ -156,6 +156,10 @@ int  ice_gpu_residual_norm(ice_gpu_ctx* ctx, double* total, double* fam);
 /* Time (ms, HIP events on the context's stream) of the last ice_gpu_step, and
  * the summed duration and launch count of the dominant kernel in it. */
 int  ice_gpu_timing(ice_gpu_ctx* ctx, double* step_ms, double* kernel_ms, int64_t* kernel_launches);
+/* Cells per workgroup tile of the stage kernel (tx*ty*tz, and the faces of the
+ * tile in each direction, at most 256); tx = -1 restores the default shapes,
+ * tx = 0 selects the untiled one-thread-per-cell kernel. */
+int  ice_gpu_set_tiles(ice_gpu_ctx* ctx, int tx, int ty, int tz);
 /* 1: bracket every launch of the dominant kernel with HIP events (bench only). */
 int  ice_gpu_set_kernel_timing(ice_gpu_ctx* ctx, int on);
 

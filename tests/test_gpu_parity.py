@@ -1,5 +1,10 @@
-"""GPU path against the CPU oracle on the same seeded inputs (FP64, 1e-12 relative
-L2 per variable after N steps)."""
+"""GPU path against the CPU oracle on the same seeded inputs.
+
+Without transcendental functions in the loop (no gas coupling) the device path
+computes the oracle's operations in the oracle's order, so states are compared
+bit for bit.  With the drag law the device's pow/exp/tanh (ROCm ocml) and the
+host's (glibc) may differ in the last bit; those runs are held to 1e-12
+relative L2 per variable after N steps."""
 import numpy as np
 import pytest
 
@@ -10,14 +15,23 @@ from helpers import compare_states, run_both
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("limiter", [1, 3, 8])
-def test_riemann_1d(gpu, oracle, limiter):
-    case = cases.case_riemann_1d(n=400, steps=40, limiter=limiter)
-    print(f"case riemann1d n=400 limiter={limiter}", flush=True)
+@pytest.mark.parametrize("limiter", list(range(11)))
+def test_riemann_1d_bit_exact(gpu, oracle, limiter):
+    case = cases.case_riemann_1d(n=400, steps=40, limiter=limiter, drag=False)
+    print(f"case riemann1d n=400 limiter={limiter} uncoupled", flush=True)
     rg, ro, sg, so = run_both(gpu, oracle, case, 40)
     assert sg.iter == so.iter == 40
+    assert sg.dtglobal == so.dtglobal and sg.time == so.time
+    a, b = rg.state(1, 1), ro.state(1, 1)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("limiter", [1, 3])
+def test_riemann_1d_drag(gpu, oracle, limiter):
+    case = cases.case_riemann_1d(n=2000, steps=100, limiter=limiter)
+    print(f"case riemann1d n=2000 limiter={limiter} drag", flush=True)
+    rg, ro, sg, so = run_both(gpu, oracle, case, 100)
     assert sg.dtglobal == so.dtglobal
-    assert abs(sg.time - so.time) <= 1e-15 * abs(so.time)
     err = compare_states(rg, ro, case)
     print("rel L2", err)
     assert err <= 1e-12
@@ -28,5 +42,18 @@ def test_crossing_jets_small(gpu, oracle):
     print("case crossing_jets n=64", flush=True)
     rg, ro, sg, so = run_both(gpu, oracle, case, 30)
     err = compare_states(rg, ro, case)
-    print("rel L2", err, "dt", sg.dtglobal, so.dtglobal)
+    print("rel L2", err)
     assert err <= 1e-12
+
+
+def test_tiled_equals_untiled(gpu):
+    """The LDS-tiled stage kernel and the one-thread-per-cell kernel give the same bits."""
+    import ctypes as C
+    for case in (cases.case_riemann_1d(n=300, steps=10, limiter=3), cases.case_crossing_jets(n=48, steps=10)):
+        print("case", case.name, flush=True)
+        ra = cases.Runner(gpu, case, "ice_gpu_")
+        rb = cases.Runner(gpu, case, "ice_gpu_")
+        assert gpu.ice_gpu_set_tiles(rb.ctx, 0, 0, 0) == 0
+        ra.step(10)
+        rb.step(10)
+        assert np.array_equal(ra.state(1, 1), rb.state(1, 1))
