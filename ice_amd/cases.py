@@ -363,6 +363,13 @@ class Runner:
         a = out.reshape((nc, ni + 4, nj + 4, nk + 4), order="F")
         return a[:, 2:ni + 2, 2:nj + 2, 2:nk + 2].copy()
 
+    def residual_norm(self):
+        """(total[5], fam[ngroups, 5]) norms of rho, u, v, w, T changes over the last step"""
+        tot = np.zeros(5)
+        fam = np.zeros(5 * self.case.cfg.ngroups)
+        self.check(self.f("residual_norm")(self.ctx, tot, fam), "residual_norm")
+        return tot, fam.reshape(self.case.cfg.ngroups, 5)
+
     def bc_maps(self, p: int):
         nrec = self.case.reci.shape[0]
         partner = np.zeros(nrec, np.int32)
@@ -387,3 +394,125 @@ class Runner:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- case 3
+def smooth_field(rng, X, Y, Z, modes: int = 16, kmax: int = 4):
+    """sum of `modes` random sine modes over the box, wave numbers 1..kmax per axis"""
+    f = np.zeros_like(X)
+    for _ in range(modes):
+        k = rng.integers(1, kmax + 1, size=3) * rng.choice([-1, 1], size=3)
+        f += rng.normal() * np.sin(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z) + rng.uniform(0, 2 * np.pi))
+    return f
+
+
+def block_tree(nblocks: int, emin: int, emax: int, rng):
+    """A seeded random tree of face-matching blocks: each new block takes a free face
+    of an existing block, copies its two edges, draws its third (log-uniform), and
+    sits against that face with the same index directions.  Returns dims, origins
+    (in cells) and the joins (a, fa, b, fb)."""
+    logu = lambda: int(round(np.exp(rng.uniform(np.log(emin), np.log(emax)))))  # noqa: E731
+    dims = [(logu(), logu(), logu())]
+    orig = [(0, 0, 0)]
+    free = [(0, f) for f in range(1, 7)]
+    joins = []
+    while len(dims) < nblocks:
+        a, fa = free.pop(int(rng.integers(len(free))))
+        d = (fa - 1) // 2
+        fb = fa + 1 if fa % 2 == 1 else fa - 1
+        nd = list(dims[a])
+        nd[d] = logu()
+        o = list(orig[a])
+        o[d] = orig[a][d] + dims[a][d] if fa % 2 == 0 else orig[a][d] - nd[d]
+        b = len(dims)
+        dims.append(tuple(nd))
+        orig.append(tuple(o))
+        joins.append((a, fa, b, fb))
+        free += [(b, f) for f in range(1, 7) if f != fb]
+    return dims, orig, joins
+
+
+def case_multiblock(nblocks: int = 64, emin: int = 16, emax: int = 128, seed: int = 20261205, steps: int = 100,
+                    ngroups: int = 3, mode: int = 0, limiter: int = 3, dx: float = 1e-3) -> Case:
+    """3-D multi-block batching: `nblocks` blocks with edges log-uniform on [emin, emax],
+    joined by face-matching connections (101) into a seeded random tree, extrapolation
+    (400) elsewhere.  IG closure, `ngroups` families with lognormal radii (median
+    5/20/60 um, sigma 0.4) giving n = rho / (rho_mat 4/3 pi r^3); rho_p ~ 1e-3 (1 +
+    0.3 noise), P = rho a^2 / 3 with a = 10 m/s.  Frozen gas: smooth random velocity
+    field (|u| <= 30 m/s) at 300-600 K.  Rusanov, MUSCL, SSP-RK3, drag, FP64."""
+    rng = np.random.default_rng(seed)
+    dims, orig, joins = block_tree(nblocks, emin, emax, rng)
+    rho_mat = 1000.0
+    medians = [5e-6, 20e-6, 60e-6][:ngroups] + [30e-6] * max(0, ngroups - 3)
+    lo = np.min(np.asarray(orig), axis=0)
+    hi = np.max(np.asarray(orig) + np.asarray(dims), axis=0)
+    span = (hi - lo).astype(float)
+    gseed = rng.integers(1 << 30)
+    blocks = []
+    for b, (dd, o) in enumerate(zip(dims, orig)):
+        ni, nj, nk = dd
+        xs = [(o[m] + np.arange(dd[m] + 1)) * dx for m in range(3)]
+        geo = geometry.metrics(geometry.cartesian_nodes(*xs))
+        c = [((o[m] - lo[m]) + np.arange(dd[m]) + 0.5) / span[m] for m in range(3)]
+        X, Y, Z = np.meshgrid(*c, indexing="ij")
+        g_rng = np.random.default_rng(gseed)  # same field for every block, sampled at the block's cells
+        u = smooth_field(g_rng, X, Y, Z)
+        v = smooth_field(g_rng, X, Y, Z)
+        w = smooth_field(g_rng, X, Y, Z)
+        t = smooth_field(g_rng, X, Y, Z)
+        blocks.append([dd, geo, (X, Y, Z), (u, v, w, t)])
+    umax = max(np.sqrt(bb[3][0] ** 2 + bb[3][1] ** 2 + bb[3][2] ** 2).max() for bb in blocks)
+    tmin = min(bb[3][3].min() for bb in blocks)
+    tmax = max(bb[3][3].max() for bb in blocks)
+    out = []
+    for b, (dd, geo, (X, Y, Z), (u, v, w, t)) in enumerate(blocks):
+        g5 = np.zeros((5,) + dd)
+        g5[0] = 1.2
+        g5[1], g5[2], g5[3] = 30.0 * u / umax, 30.0 * v / umax, 30.0 * w / umax
+        g5[4] = 300.0 + 300.0 * (t - tmin) / (tmax - tmin)
+        gas = gas_arrays(g5, 287.05, 1.4, 0.026, 1.8e-5)
+        prims = []
+        for p in range(ngroups):
+            prng = np.random.default_rng([seed, b, p])
+
+            def fill(I, J, K, prng=prng, p=p):
+                shp = I.shape
+                rho = 1e-3 * (1.0 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) * np.sin(2 * np.pi * Z))
+                r = medians[p] * np.exp(0.4 * prng.standard_normal(shp))
+                num = rho / (rho_mat * 4.0 / 3.0 * PI * r ** 3)
+                z = np.zeros(shp)
+                return np.stack([rho, g5[1] * 0.9, g5[2] * 0.9, g5[3] * 0.9, rho * 100.0 / 3.0, z + 300.0, num])
+
+            prims.append(padded_state(dd, 7, fill))
+        out.append(Block(dd, geo, prims, gas))
+    # boundary table: 101 across the joins, 400 elsewhere
+    partner_face = {}
+    for a, fa, b, fb in joins:
+        partner_face[(a, fa)] = (b, fb)
+        partner_face[(b, fb)] = (a, fa)
+    tb = TableBuilder(ngroups)
+    for b, dd in enumerate(dims):
+        def codes(f, i, j, k, b=b):
+            return np.array([101 if (b, int(ff)) in partner_face else 400 for ff in f], np.int32)
+
+        def ints(ri, fijk, b=b, dd=dd):
+            for f in range(1, 7):
+                if (b, f) not in partner_face:
+                    continue
+                ob, of = partner_face[(b, f)]
+                m = fijk[:, 0] == f
+                d = (f - 1) // 2
+                src = fijk[m, 1:4].copy()
+                # the cell across the face: same tangential indices, first interior cell of the other block
+                src[:, d] = 1 if of % 2 == 1 else dims[ob][d]
+                ri[m, R_BS] = ob + 1
+                ri[m, R_IS], ri[m, R_JS], ri[m, R_KS] = src[:, 0], src[:, 1], src[:, 2]
+                ri[m, R_FS] = of
+
+        tb.add_block(b + 1, dd, codes, ints_fn=ints)
+    reci, recd, donor, weight = tb.arrays()
+    cfg = make_config(ngroups=ngroups, nrk=3, time_accurate=True, ndir=3, limiter=limiter, coupled=True,
+                      drag_select=4, heat_select=8, cfl=0.5, mode=mode)
+    fams = [make_family(IG, RUSANOV, rho_mat=rho_mat, cs=4186.0, lv=2.26e6) for _ in range(ngroups)]
+    return Case("multiblock", cfg, fams, out, reci, recd, donor, weight, steps,
+                meta={"joins": joins, "origins": orig})

@@ -66,6 +66,7 @@ struct ice_gpu_ctx {
   double step_ms = 0.0, kernel_ms = 0.0;
   long long kernel_launches = 0;
   std::vector<hipEvent_t> kev;
+  std::vector<FamArgs*> kev_args;  // FamArgs arrays of eager batched steps, freed with the events
   BatchedPlan* batched = nullptr;
   int tile[3] = {-1, -1, -1};  // cells per stage tile; -1 default shape, 0 per-cell kernel
   std::string err;
@@ -298,7 +299,10 @@ FamArgs fam_args(ice_gpu_ctx* c, int p) {
 
 // ---------------------------------------------------------------- dispatch
 __global__ void k_step_begin(DevStatus* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) st->dtbits = (unsigned long long)__double_as_longlong(1e+5);
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->dtbits = (unsigned long long)__double_as_longlong(1e+5);
+    st->iter = st->iter + 1;
+  }
 }
 
 __global__ void k_dt_end(DevStatus* st, int time_accurate) {
@@ -307,6 +311,54 @@ __global__ void k_dt_end(DevStatus* st, int time_accurate) {
     st->dtglobal = dt;
     if (time_accurate) st->time = st->time + dt;
   }
+}
+
+// residual_norm partial sums: workgroup w of block b strides over the block's
+// interior cells and writes the five sums of squared changes of rho, u, v, w, T
+// (state `q` against the step's start state `o`), reduced in a fixed tree order.
+constexpr int kResWG = 64;  // workgroups per (block, family)
+__global__ __launch_bounds__(256) void k_resnorm_part(const BlkDev* __restrict__ blk, const double* __restrict__ q,
+                                                      const double* __restrict__ o, long long N, int it,
+                                                      double* __restrict__ part, int ngroups, int p) {
+  const int b = blockIdx.y;
+  const BlkDev B = blk[b];
+  const long long cells = (long long)B.ni * B.nj * B.nk;
+  double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cells; t += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(t % B.ni) + 1;
+    long long r = t / B.ni;
+    int j = (int)(r % B.nj) + 1;
+    int k = (int)(r / B.nj) + 1;
+    long long g = gidx(B, i, j, k);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double d = fabs(q[(long long)m * N + g] - o[(long long)m * N + g]);
+      s[m] += d * d;
+    }
+    double d = fabs(q[(long long)it * N + g] - o[(long long)it * N + g]);
+    s[4] += d * d;
+  }
+  __shared__ double red[5][256];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) red[m][threadIdx.x] = s[m];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) part[(((long long)b * ngroups + p) * kResWG + blockIdx.x) * 5 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// the kResWG partials of every (block, family) summed in workgroup order
+__global__ void k_resnorm_sum(const double* __restrict__ part, double* __restrict__ out, int npairs) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs * 5) return;
+  int pair = t / 5, m = t % 5;
+  double s = 0.0;
+  for (int w = 0; w < kResWG; ++w) s += part[((long long)pair * kResWG + w) * 5 + m];
+  out[t] = s;
 }
 
 void launch_dt(ice_gpu_ctx* c, int p, const FamArgs& a) {
@@ -719,6 +771,8 @@ int ice_gpu_step(ice_gpu_ctx* c, int nsteps, ice_status_t* st) {
   if (c->rec.empty()) return fail(c, "[ICE::step] no boundary table: call ice_gpu_set_boundaries first");
   for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
   c->kev.clear();
+  for (FamArgs* d : c->kev_args) (void)hipFree(d);
+  c->kev_args.clear();
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   for (int s = 0; s < nsteps; ++s) {
     int rc = (c->cfg.mode == ICE_MODE_BATCHED) ? batched_step(c) : plain_step(c);
@@ -777,6 +831,42 @@ int ice_gpu_get_source(ice_gpu_ctx* c, int b, int p, double* source, double* tau
         for (int v = 0; v < F.nc; ++v) source[s * F.nc + v] = h[(size_t)v * npad + g];
         tau[s] = h[(size_t)F.nc * npad + g];
       }
+  return 0;
+}
+
+int ice_gpu_residual_norm(ice_gpu_ctx* c, double* total, double* fam) {
+  if (!c) return 1;
+  if (!total || !fam) return fail(c, "[ICE::residual_norm] null output");
+  const int nb = (int)c->blk.size(), ng = c->cfg.ngroups;
+  double *part = nullptr, *sums = nullptr;
+  if (dalloc(c, &part, (size_t)nb * ng * kResWG * 5) || dalloc(c, &sums, (size_t)nb * ng * 5)) {
+    if (part) (void)hipFree(part);
+    return 1;
+  }
+  for (int p = 0; p < ng; ++p) {
+    const FamDev& F = c->fam[p];
+    hipLaunchKernelGGL(k_resnorm_part, dim3(kResWG, nb), dim3(256), 0, c->stream, c->d_blk, F.buf[F.cur],
+                       F.buf[F.old], c->N, F.nc - 2, part, ng, p);
+  }
+  hipLaunchKernelGGL(k_resnorm_sum, dim3((nb * ng * 5 + 255) / 256), dim3(256), 0, c->stream, part, sums, nb * ng);
+  std::vector<double> h((size_t)nb * ng * 5);
+  hipError_t e = hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(part);
+  (void)hipFree(sums);
+  if (e != hipSuccess) return fail(c, "[ICE::residual_norm] %s", hipGetErrorString(e));
+  // totals over blocks then families, family sums over blocks (Mod_Diagnostic.f90:106-127)
+  for (int m = 0; m < 5; ++m) total[m] = 0.0;
+  for (int m = 0; m < 5 * ng; ++m) fam[m] = 0.0;
+  for (int b = 0; b < nb; ++b)
+    for (int p = 0; p < ng; ++p)
+      for (int m = 0; m < 5; ++m) {
+        double x = h[((size_t)b * ng + p) * 5 + m];
+        total[m] = total[m] + x;
+        fam[p * 5 + m] = fam[p * 5 + m] + x;
+      }
+  for (int m = 0; m < 5; ++m) total[m] = std::sqrt(total[m]);
+  for (int m = 0; m < 5 * ng; ++m) fam[m] = std::sqrt(fam[m]);
   return 0;
 }
 
@@ -843,6 +933,249 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
 
 }  // extern "C"
 
-// Batched path: not yet built; the mode refuses to run rather than fall back.
-int batched_step(ice_gpu_ctx* c) { return fail(c, "[ICE::step] the batched mode is not available in this build"); }
-void batched_destroy(ice_gpu_ctx*) {}
+// ================================================================= batched path
+// All blocks and families of a stage in one persistent kernel per closure
+// group, fed from per-XCD tile queues; the step's whole launch sequence is
+// captured once per buffer rotation in a hipGraph and replayed.
+struct BatchGroup {
+  int closure = 0, riemann = 0;
+  std::vector<int> fams;                 // 0-based family numbers
+  WorkItem* d_items = nullptr;
+  int* d_qstart = nullptr;
+  unsigned long long* d_ticket = nullptr;
+  int nitems = 0, nq = 8, grid = 0;
+};
+
+struct BatchGraph {
+  int key = -1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<FamArgs*> d_args;           // device FamArgs arrays owned by the graph
+  std::vector<int> end_cur;              // each family's state buffer after the step
+};
+
+struct BatchedPlan {
+  std::vector<BatchGroup> groups;
+  std::vector<BatchGraph> graphs;
+  int cus = 0;
+};
+
+namespace {
+
+int build_plan(ice_gpu_ctx* c) {
+  BatchedPlan* P = new BatchedPlan();
+  c->batched = P;
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, c->device));
+  P->cus = prop.multiProcessorCount;
+  for (int p = 0; p < c->cfg.ngroups; ++p) {
+    int cl = c->fams[p].closure, rs = c->fams[p].riemann;
+    BatchGroup* g = nullptr;
+    for (auto& x : P->groups)
+      if (x.closure == cl && x.riemann == rs) g = &x;
+    if (!g) {
+      P->groups.emplace_back();
+      g = &P->groups.back();
+      g->closure = cl;
+      g->riemann = rs;
+    }
+    g->fams.push_back(p);
+  }
+  for (auto& g : P->groups) {
+    // items block by block, tile by tile, the group's families side by side so a
+    // tile's geometry and gas are read from L2 by all of them
+    std::vector<WorkItem> items;
+    for (size_t b = 0; b < c->blk.size(); ++b) {
+      Tile T = stage_tile(c, c->blk[b]);
+      if (T.tx == 0) return fail(c, "[ICE::batched] the batched mode needs a tiled stage kernel");
+      int nt = T.ntx * T.nty * T.ntz;
+      for (int t = 0; t < nt; ++t)
+        for (int p : g.fams) {
+          WorkItem w;
+          w.fam = p;
+          w.block = (int)b;
+          w.tile = t;
+          w.pad = 0;
+          w.T = T;
+          items.push_back(w);
+        }
+    }
+    g.nitems = (int)items.size();
+    g.nq = 8;
+    std::vector<int> qs(g.nq + 1);
+    for (int q = 0; q <= g.nq; ++q) qs[q] = (int)((long long)g.nitems * q / g.nq);
+    if (dalloc(c, &g.d_items, items.size()) || dalloc(c, &g.d_qstart, qs.size()) || dalloc(c, &g.d_ticket, g.nq))
+      return 1;
+    HIPCHK(hipMemcpy(g.d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g.d_qstart, qs.data(), qs.size() * sizeof(int), hipMemcpyHostToDevice));
+    int wgs = g.closure == ICE_MK ? persistent_wgs_mk(P->cus) : (g.closure == ICE_IG ? persistent_wgs_ig(P->cus)
+                                                                                       : persistent_wgs_ag(P->cus));
+    g.grid = std::max(g.nq, (wgs / g.nq) * g.nq);
+  }
+  return 0;
+}
+
+// The FamArgs of every pass of one step with the current buffer roles (index 0
+// the dt pass, then one per RK stage), uploaded; the roles after the step.
+struct StepArgs {
+  std::vector<std::vector<FamArgs>> h;
+  std::vector<FamArgs*> d;
+  std::vector<int> end_cur, start_cur;
+};
+
+int prepare_step_args(ice_gpu_ctx* c, StepArgs& A, std::vector<FamArgs*>& owner) {
+  int ng = c->cfg.ngroups, nrk = c->cfg.nrk;
+  std::vector<int> cur(ng), old(ng);
+  for (int p = 0; p < ng; ++p) cur[p] = old[p] = c->fam[p].cur;
+  A.start_cur = cur;
+  A.h.assign(nrk + 1, std::vector<FamArgs>(ng));
+  for (int pass = 0; pass <= nrk; ++pass) {
+    std::vector<int> nxt(ng);
+    for (int p = 0; p < ng; ++p) {
+      FamDev& F = c->fam[p];
+      int kc = F.cur, ko = F.old;
+      F.cur = cur[p];
+      F.old = old[p];
+      FamArgs a = fam_args(c, p);
+      if (pass > 0) {
+        nxt[p] = other_buffer(cur[p], old[p]);
+        a.nxt = F.buf[nxt[p]];
+        a.srk = pass;
+        a.last_stage = pass == nrk;
+      }
+      A.h[pass][p] = a;
+      F.cur = kc;
+      F.old = ko;
+    }
+    if (pass > 0) cur = nxt;
+  }
+  A.end_cur = cur;
+  A.d.assign(nrk + 1, nullptr);
+  for (int pass = 0; pass <= nrk; ++pass) {
+    if (dalloc(c, &A.d[pass], ng)) return 1;
+    owner.push_back(A.d[pass]);
+    HIPCHK(hipMemcpy(A.d[pass], A.h[pass].data(), ng * sizeof(FamArgs), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+BatchLaunch group_launch(ice_gpu_ctx* c, const BatchGroup& g, const std::vector<FamArgs>& h, FamArgs* d,
+                         std::vector<StageLaunch>& per) {
+  BatchLaunch B;
+  B.stream = c->stream;
+  B.d_fams = d;
+  B.h_fams = h.data();
+  B.fam_ids = g.fams.data();
+  B.nfam = (int)g.fams.size();
+  per.resize(g.fams.size());
+  for (size_t f = 0; f < g.fams.size(); ++f) {
+    const FamDev& F = c->fam[g.fams[f]];
+    StageLaunch& L = per[f];
+    std::memset(&L, 0, sizeof L);
+    L.stream = c->stream;
+    L.grp = F.grp; L.ngrp = F.ngrp; L.own = F.own; L.nown = F.nown; L.nbcell = F.nbcell;
+  }
+  B.per_fam = per.data();
+  B.Q.items = g.d_items;
+  B.Q.qstart = g.d_qstart;
+  B.Q.ticket = g.d_ticket;
+  B.Q.nq = g.nq;
+  B.grid = g.grid;
+  B.riemann = g.riemann;
+  B.sd = c->cfg.shock_detector;
+  B.coupled = c->cfg.coupled;
+  B.limiter = c->cfg.limiter;
+  return B;
+}
+
+// the launches of one step (no allocation: capturable)
+void launch_batched_step(ice_gpu_ctx* c, const StepArgs& A, bool timed) {
+  BatchedPlan* P = c->batched;
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, c->stream, c->d_st);
+  for (auto& g : P->groups) {
+    std::vector<StageLaunch> per;
+    BatchLaunch B = group_launch(c, g, A.h[0], A.d[0], per);
+    if (g.closure == ICE_MK) launch_batched_dt_mk(B);
+    else if (g.closure == ICE_IG) launch_batched_dt_ig(B);
+    else launch_batched_dt_ag(B);
+  }
+  hipLaunchKernelGGL(k_dt_end, dim3(1), dim3(64), 0, c->stream, c->d_st, c->cfg.time_accurate);
+  for (int srk = 1; srk <= c->cfg.nrk; ++srk) {
+    for (auto& g : P->groups) {
+      std::vector<StageLaunch> per;
+      BatchLaunch B = group_launch(c, g, A.h[srk], A.d[srk], per);
+      if (timed) {
+        (void)hipEventCreate(&B.ev0);
+        (void)hipEventCreate(&B.ev1);
+      }
+      if (g.closure == ICE_MK) launch_batched_mk(B);
+      else if (g.closure == ICE_IG) launch_batched_ig(B);
+      else launch_batched_ag(B);
+      if (timed) {
+        c->kev.push_back(B.ev0);
+        c->kev.push_back(B.ev1);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int batched_step(ice_gpu_ctx* c) {
+  if (c->cfg.irs) return fail(c, "[ICE::step] the batched mode does not smooth residuals (irs); use the plain mode");
+  if (!c->batched && build_plan(c)) return 1;
+  BatchedPlan* P = c->batched;
+  c->iter += 1;
+  const int ng = c->cfg.ngroups;
+  if (c->kernel_timing) {
+    StepArgs A;
+    if (prepare_step_args(c, A, c->kev_args)) return 1;
+    launch_batched_step(c, A, true);
+    for (int p = 0; p < ng; ++p) {
+      c->fam[p].old = c->fam[p].cur;
+      c->fam[p].cur = A.end_cur[p];
+    }
+    return 0;
+  }
+  int key = 0;
+  for (int p = 0; p < ng; ++p) key = key * 3 + c->fam[p].cur;
+  BatchGraph* G = nullptr;
+  for (auto& x : P->graphs)
+    if (x.key == key) G = &x;
+  if (!G) {
+    P->graphs.emplace_back();
+    G = &P->graphs.back();
+    G->key = key;
+    StepArgs A;
+    if (prepare_step_args(c, A, G->d_args)) return 1;
+    G->end_cur = A.end_cur;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
+    launch_batched_step(c, A, false);
+    hipError_t e = hipStreamEndCapture(c->stream, &G->graph);
+    if (e != hipSuccess) return fail(c, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    HIPCHK(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+  }
+  HIPCHK(hipGraphLaunch(G->exec, c->stream));
+  for (int p = 0; p < ng; ++p) {
+    c->fam[p].old = c->fam[p].cur;
+    c->fam[p].cur = G->end_cur[p];
+  }
+  return 0;
+}
+
+void batched_destroy(ice_gpu_ctx* c) {
+  BatchedPlan* P = c->batched;
+  if (!P) return;
+  for (auto& g : P->groups) {
+    if (g.d_items) (void)hipFree(g.d_items);
+    if (g.d_qstart) (void)hipFree(g.d_qstart);
+    if (g.d_ticket) (void)hipFree(g.d_ticket);
+  }
+  for (auto& G : P->graphs) {
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    if (G.graph) (void)hipGraphDestroy(G.graph);
+    for (FamArgs* d : G.d_args) (void)hipFree(d);
+  }
+  delete P;
+  c->batched = nullptr;
+}

@@ -34,7 +34,7 @@ struct DevStatus {
   unsigned long long n_floored;
   int bad_state, bad_recon;
   int bad_b, bad_i, bad_j, bad_k, bad_p;
-  int pad;
+  int iter;                   // grid%iter of the step being computed
 };
 
 // Everything a kernel of one family reads or writes

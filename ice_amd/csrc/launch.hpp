@@ -106,6 +106,93 @@ template <int M> void launch_dt_kernel(hipStream_t stream, const BlkDev* blk, in
     hipLaunchKernelGGL(k_dt<M>, dim3(grid_cells(blk[b].cells)), dim3(kThreads), 0, stream, a, b);
 }
 
+// ------------------------------------------------------------------ batched path
+// One closure group (families of one closure and Riemann solver) of the batched
+// step: its families' record kernels, then the persistent kernels over the
+// group's work queues.
+struct BatchLaunch {
+  hipStream_t stream;
+  const FamArgs* d_fams;       // device copies, indexed by WorkItem::fam (family number - 1)
+  const FamArgs* h_fams;       // host copies of the same
+  const int* fam_ids;          // the group's families
+  int nfam;
+  const StageLaunch* per_fam;  // record lists of each family of the group
+  WorkQueues Q;
+  int grid;                    // persistent workgroups
+  int riemann, sd, coupled, limiter;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // recorded around the persistent stage kernel when set
+};
+
+inline void reset_tickets(const BatchLaunch& B) {
+  hipLaunchKernelGGL(k_zero_tickets, dim3(1), dim3(64), 0, B.stream, B.Q.ticket, B.Q.nq);
+}
+
+template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchLaunch& B) {
+  for (int f = 0; f < B.nfam; ++f) {
+    const StageLaunch& L = B.per_fam[f];
+    FamArgs a = B.h_fams[B.fam_ids[f]];
+    a.list = L.grp;
+    a.nlist = L.ngrp;
+    if (L.ngrp) {
+      hipLaunchKernelGGL(k_ghost1<M>, dim3(grid_cells(L.ngrp)), dim3(kThreads), 0, B.stream, a);
+      hipLaunchKernelGGL(k_ghost2<M>, dim3(grid_cells(L.ngrp)), dim3(kThreads), 0, B.stream, a);
+    }
+  }
+  if (SD) {
+    reset_tickets(B);
+    hipLaunchKernelGGL(k_beta_persist<M>, dim3(B.grid), dim3(kThreads), 0, B.stream, B.d_fams, B.Q);
+  }
+  for (int f = 0; f < B.nfam; ++f) {
+    const StageLaunch& L = B.per_fam[f];
+    FamArgs a = B.h_fams[B.fam_ids[f]];
+    if (L.nown) {
+      a.list = L.own;
+      a.nlist = L.nown;
+      hipLaunchKernelGGL((k_conn<M, S, SD>), dim3(grid_cells(L.nown)), dim3(kThreads), 0, B.stream, a);
+    }
+    if (L.nbcell) hipLaunchKernelGGL((k_bcell<M, S, SD>), dim3(grid_cells(L.nbcell)), dim3(kThreads), 0, B.stream, a);
+  }
+  reset_tickets(B);
+  if (B.ev0) (void)hipEventRecord(B.ev0, B.stream);
+  dim3 grid(B.grid), blk(kThreads);
+  if (B.limiter == 1) hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, 1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
+  else if (B.limiter == 3) hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q);
+  else hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, -1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
+  if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
+}
+
+template <int M, int S> void launch_batched_sd_cp(const BatchLaunch& B) {
+  if (B.sd && B.coupled) launch_batched_stage<M, S, true, true>(B);
+  else if (B.sd) launch_batched_stage<M, S, true, false>(B);
+  else if (B.coupled) launch_batched_stage<M, S, false, true>(B);
+  else launch_batched_stage<M, S, false, false>(B);
+}
+
+template <int M> void launch_batched_dt_kernel(const BatchLaunch& B) {
+  reset_tickets(B);
+  hipLaunchKernelGGL(k_dt_persist<M>, dim3(B.grid), dim3(kThreads), 0, B.stream, B.d_fams, B.Q);
+}
+
+// persistent workgroups of the stage kernel on this device
+template <int M, int S, bool SD, bool CP> int persistent_wgs_of(int cus) {
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_stage_persist<M, S, SD, CP, false, -1>,
+                                                   kThreads, 0) != hipSuccess || per < 1)
+    per = 1;
+  return cus * per;
+}
+
+// defined in stage_mk.hip, stage_ig.hip, stage_ag.hip
+void launch_batched_mk(const BatchLaunch& B);
+void launch_batched_ig(const BatchLaunch& B);
+void launch_batched_ag(const BatchLaunch& B);
+void launch_batched_dt_mk(const BatchLaunch& B);
+void launch_batched_dt_ig(const BatchLaunch& B);
+void launch_batched_dt_ag(const BatchLaunch& B);
+int persistent_wgs_mk(int cus);
+int persistent_wgs_ig(int cus);
+int persistent_wgs_ag(int cus);
+
 // defined in stage_mk.hip, stage_ig.hip, stage_ag.hip
 void launch_stage_mk(const StageLaunch& L, const FamArgs& a);
 void launch_stage_ig(const StageLaunch& L, const FamArgs& a);

@@ -7,4 +7,10 @@ void launch_stage_ag(const StageLaunch& L, const FamArgs& a) {
   else launch_sd_cp<2, 1>(L, a);
 }
 void launch_dt_ag(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<2>(s, blk, nb, a); }
+void launch_batched_ag(const BatchLaunch& B) {
+  if (B.riemann == 2) launch_batched_sd_cp<2, 2>(B);
+  else launch_batched_sd_cp<2, 1>(B);
+}
+void launch_batched_dt_ag(const BatchLaunch& B) { launch_batched_dt_kernel<2>(B); }
+int persistent_wgs_ag(int cus) { return persistent_wgs_of<2, 1, true, true>(cus); }
 }  // namespace ice

@@ -7,4 +7,10 @@ void launch_stage_ig(const StageLaunch& L, const FamArgs& a) {
   else launch_sd_cp<1, 1>(L, a);
 }
 void launch_dt_ig(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<1>(s, blk, nb, a); }
+void launch_batched_ig(const BatchLaunch& B) {
+  if (B.riemann == 2) launch_batched_sd_cp<1, 2>(B);
+  else launch_batched_sd_cp<1, 1>(B);
+}
+void launch_batched_dt_ig(const BatchLaunch& B) { launch_batched_dt_kernel<1>(B); }
+int persistent_wgs_ig(int cus) { return persistent_wgs_of<1, 1, true, true>(cus); }
 }  // namespace ice

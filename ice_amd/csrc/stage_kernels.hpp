@@ -35,38 +35,43 @@ ICE_DEV void block_min_atomic(double v, unsigned long long* target) {
 // ------------------------------------------------------------------ dt
 // compute_dt / compute_dt_ (Mod_dt.f90:11-112, 256-275)
 template <int M>
-__global__ void __launch_bounds__(kThreads) k_dt(FamArgs a, int b) {
+ICE_DEV double dt_cell(const FamArgs& a, long long g) {
   constexpr int NC = Closure<M>::NC;
+  double q[NC];
+  load_cell<NC>(a.cur, a.N, g, q);
+  double dt = 1e+5;
+  double rate = 0.0;
+  for (int d = 0; d < a.ndir; ++d) {
+    double r0 = a.M[(long long)(d * 3 + 0) * a.N + g];
+    double r1 = a.M[(long long)(d * 3 + 1) * a.N + g];
+    double r2 = a.M[(long long)(d * 3 + 2) * a.N + g];
+    double gn = norm3(r0, r1, r2);
+    double vs[3] = {r0 / gn, r1 / gn, r2 / gn};
+    double un = q[1] * vs[0] + q[2] * vs[1] + q[3] * vs[2];
+    rate = rate + (fabs(un) + wavespeed<M>(q, vs)) * gn;
+  }
+  if (rate > 0.0) dt = dmin2(dt, 1.0 / rate);
+  dt = dt * a.cfl;
+  int iter = a.st->iter;
+  if (iter <= a.cfl_ramp_iter) dt = dt * (double)iter / (double)a.cfl_ramp_iter;
+  dt = dmin2(dt, a.dt_max);
+  if (a.coupled && a.tau_factor > 0.0) {
+    double tau = a.tau[g];
+    if (q[0] > kRhoEmpty && !nonfinite(tau) && tau > 0.0) dt = dmin2(dt, a.tau_factor * tau);
+  }
+  if (!a.time_accurate) a.dt[g] = dt;
+  return dt;
+}
+
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_dt(FamArgs a, int b) {
   const BlkDev B = a.blk[b];
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double dtc = 1.79769313486231571e308;
   if (t < B.cells) {
     int i, j, k;
     cell_of(B, t, i, j, k);
-    long long g = gidx(B, i, j, k);
-    double q[NC];
-    load_cell<NC>(a.cur, a.N, g, q);
-    double dt = 1e+5;
-    double rate = 0.0;
-    for (int d = 0; d < a.ndir; ++d) {
-      double r0 = a.M[(long long)(d * 3 + 0) * a.N + g];
-      double r1 = a.M[(long long)(d * 3 + 1) * a.N + g];
-      double r2 = a.M[(long long)(d * 3 + 2) * a.N + g];
-      double gn = norm3(r0, r1, r2);
-      double vs[3] = {r0 / gn, r1 / gn, r2 / gn};
-      double un = q[1] * vs[0] + q[2] * vs[1] + q[3] * vs[2];
-      rate = rate + (fabs(un) + wavespeed<M>(q, vs)) * gn;
-    }
-    if (rate > 0.0) dt = dmin2(dt, 1.0 / rate);
-    dt = dt * a.cfl;
-    if (a.iter <= a.cfl_ramp_iter) dt = dt * (double)a.iter / (double)a.cfl_ramp_iter;
-    dt = dmin2(dt, a.dt_max);
-    if (a.coupled && a.tau_factor > 0.0) {
-      double tau = a.tau[g];
-      if (q[0] > kRhoEmpty && !nonfinite(tau) && tau > 0.0) dt = dmin2(dt, a.tau_factor * tau);
-    }
-    if (!a.time_accurate) a.dt[g] = dt;
-    dtc = dt;
+    dtc = dt_cell<M>(a, gidx(B, i, j, k));
   }
   block_min_atomic(dtc, &a.st->dtbits);
 }
@@ -337,57 +342,47 @@ __global__ void __launch_bounds__(kThreads) k_beta(FamArgs a, int b) {
 }
 
 // ------------------------------------------------------------------ boundary fluxes
-// record_flux (Mod_BC_Fluxes.f90:180-272)
+// record_flux (Mod_BC_Fluxes.f90:180-272).  The four stencil cells lie on one
+// grid line: (g2, g1, m, m+1) on an odd face, (m-1, m, g1, g2) on an even one,
+// so the reconstruction reads them from memory with the direction's stride.
 template <int M, int S>
 ICE_DEV bool record_flux(const FamArgs& a, const DRec& r, double beta_val, double* flux) {
   constexpr int NC = Closure<M>::NC;
   const BlkDev B = a.blk[r.b - 1];
   const long long N = a.N;
-  const int* G = kGuide[r.f - 1];
   int dir = (r.f + 1) / 2 - 1;
+  long long s = dir == 0 ? 1ll : (dir == 1 ? (long long)B.NI : (long long)B.NI * (long long)B.NJ);
+  bool odd = r.f % 2 == 1;
   long long gm = gidx(B, r.i, r.j, r.k);
-  long long g1 = gidx(B, r.i - G[0], r.j - G[1], r.k - G[2]);
-  long long g2 = gidx(B, r.i - 2 * G[0], r.j - 2 * G[1], r.k - 2 * G[2]);
-  long long gp = gidx(B, r.i + G[0], r.j + G[1], r.k + G[2]);
+  long long g1 = odd ? gm - s : gm + s;
+  long long gp = odd ? gm + s : gm - s;
   double dl_g1 = a.dl[(long long)dir * N + g1];
   double dl_m = a.dl[(long long)dir * N + gm];
   double dl_4 = a.dl[(long long)dir * N + gp];
   bool inj_g = a.inj[g1], inj_p = a.inj[gp];
-  double s0[NC], s1[NC], s2[NC], s3[NC], ql[NC], qr[NC], fl[NC];
   const double* fa = a.fa + (long long)(dir * 4) * N;
-  bool ok;
-  if (r.f % 2 == 1) {
-    long long gf = g1;
-    double nrm[3] = {fa[gf], fa[N + gf], fa[2 * N + gf]};
-    double area = fa[3 * N + gf];
-    double dl0 = dl_g1, dl1 = 0.5 * (dl_g1 + dl_m), dl2 = 0.5 * (dl_m + dl_4);
-    double dll = 0.5 * dl_g1, dlr = 0.5 * dl_m;
+  long long gf = odd ? g1 : gm;
+  double nrm[3] = {fa[gf], fa[N + gf], fa[2 * N + gf]};
+  double area = fa[3 * N + gf];
+  double dl0, dl1, dl2, dll, dlr;
+  if (odd) {
+    dl0 = dl_g1; dl1 = 0.5 * (dl_g1 + dl_m); dl2 = 0.5 * (dl_m + dl_4);
+    dll = 0.5 * dl_g1; dlr = 0.5 * dl_m;
     if (inj_g) { dl1 = 0.5 * dl_m; dll = 0.0; }
     if (inj_p) dl2 = 0.5 * dl_m;
-    load_cell<NC>(a.cur, N, g2, s0);
-    load_cell<NC>(a.cur, N, g1, s1);
-    load_cell<NC>(a.cur, N, gm, s2);
-    load_cell<NC>(a.cur, N, gp, s3);
-    ok = reconstruct<M>(a.limiter, s0, s1, s2, s3, dl0, dl1, dl2, dll, dlr, beta_val, ql, qr);
-    riemann<M, S>(ql, qr, nrm, fl, a.mat);
-#pragma unroll
-    for (int v = 0; v < NC; ++v) flux[v] = fl[v] * area;
   } else {
-    long long gf = gm;
-    double nrm[3] = {fa[gf], fa[N + gf], fa[2 * N + gf]};
-    double area = fa[3 * N + gf];
-    double dl0 = 0.5 * (dl_4 + dl_m), dl1 = 0.5 * (dl_m + dl_g1), dl2 = dl_g1;
-    double dll = 0.5 * dl_m, dlr = 0.5 * dl_g1;
+    dl0 = 0.5 * (dl_4 + dl_m); dl1 = 0.5 * (dl_m + dl_g1); dl2 = dl_g1;
+    dll = 0.5 * dl_m; dlr = 0.5 * dl_g1;
     if (inj_g) { dl1 = 0.5 * dl_m; dlr = 0.0; }
     if (inj_p) dl0 = 0.5 * dl_m;
-    load_cell<NC>(a.cur, N, gp, s0);
-    load_cell<NC>(a.cur, N, gm, s1);
-    load_cell<NC>(a.cur, N, g1, s2);
-    load_cell<NC>(a.cur, N, g2, s3);
-    ok = reconstruct<M>(a.limiter, s0, s1, s2, s3, dl0, dl1, dl2, dll, dlr, beta_val, ql, qr);
-    riemann<M, S>(ql, qr, nrm, fl, a.mat);
+  }
+  double ql[NC], qr[NC];
+  bool ok = reconstruct_mem<M>(a.limiter, a.cur + (odd ? gm - 2 * s : gm - s), s, N, dl0, dl1, dl2, dll, dlr, beta_val,
+                               ql, qr);
+  riemann_acc<M, S, 2>(ql, qr, nrm, area, flux, a.mat);
+  if (!odd) {
 #pragma unroll
-    for (int v = 0; v < NC; ++v) flux[v] = -(fl[v] * area);
+    for (int v = 0; v < NC; ++v) flux[v] = -flux[v];
   }
   return ok;
 }
@@ -602,13 +597,14 @@ constexpr int kMaxTileFaces = kThreads;
 // The stage with the interior faces shared through LDS (Mod_Fluxes.f90:421-521:
 // each face once, each cell its two faces per direction in the odd-then-even
 // order), then the cell's source, residual and update as in k_stage.
+// One tile of the stage: the body of k_stage_tiled, also run by the persistent
+// kernel for the tiles it takes from its work queue.  fl is the workgroup's
+// LDS buffer of face fluxes.
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
-__global__ void __launch_bounds__(kThreads) k_stage_tiled(FamArgs a, int b, Tile T) {
+ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl) {
   constexpr int NC = Closure<M>::NC;
-  __shared__ double fl[NC * kMaxTileFaces];
   const BlkDev& Bref = a.blk[b];
   const BlkDev B = Bref;
-  int tile = blockIdx.x;
   int ox = (tile % T.ntx) * T.tx + 1;
   int oy = ((tile / T.ntx) % T.nty) * T.ty + 1;
   int oz = (tile / (T.ntx * T.nty)) * T.tz + 1;
@@ -704,5 +700,113 @@ __global__ void __launch_bounds__(kThreads) k_stage_tiled(FamArgs a, int b, Tile
     update_cell<M>(a, Bref, ci, cj, ck, g, r);
   }
 }
+
+template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
+__global__ void __launch_bounds__(kThreads) k_stage_tiled(FamArgs a, int b, Tile T) {
+  constexpr int NC = Closure<M>::NC;
+  __shared__ double fl[NC * kMaxTileFaces];
+  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(a, b, T, blockIdx.x, fl);
+}
+
+// ------------------------------------------------------------------ batched path
+// A unit of work of the persistent kernels: one tile of one block of one family.
+struct WorkItem {
+  int fam, block, tile, pad;
+  Tile T;
+};
+
+// Work queues, one per XCD: workgroup w serves queue w % nq (the hardware places
+// consecutive workgroups on consecutive XCDs, so a queue's tiles, which are
+// contiguous in the grid, share that XCD's L2), taking items in order until the
+// queue is empty.  Every wave reaches the exit: a workgroup leaves at its first
+// ticket past the end of its queue.
+struct WorkQueues {
+  const WorkItem* items;
+  const int* qstart;               // nq + 1 offsets into items
+  unsigned long long* ticket;      // nq counters, zeroed before each launch
+  int nq;
+};
+
+ICE_DEV int next_item(const WorkQueues& Q, int q, int* shared_slot) {
+  if (threadIdx.x == 0) {
+    unsigned long long t = atomicAdd(&Q.ticket[q], 1ull);
+    long long it = (long long)Q.qstart[q] + (long long)t;
+    *shared_slot = it < Q.qstart[q + 1] ? (int)it : -1;
+  }
+  __syncthreads();
+  int it = *shared_slot;
+  __syncthreads();
+  return it;
+}
+
+template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
+__global__ void __launch_bounds__(kThreads) k_stage_persist(const FamArgs* fams, WorkQueues Q) {
+  constexpr int NC = Closure<M>::NC;
+  __shared__ double fl[NC * kMaxTileFaces];
+  __shared__ int slot;
+  int q = blockIdx.x % Q.nq;
+  for (;;) {
+    int it = next_item(Q, q, &slot);
+    if (it < 0) break;
+    const WorkItem w = Q.items[it];
+    stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl);
+  }
+}
+
+// the sensor weights and the dt pass over the same queues, one cell per thread
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_beta_persist(const FamArgs* fams, WorkQueues Q) {
+  __shared__ int slot;
+  int q = blockIdx.x % Q.nq;
+  for (;;) {
+    int it = next_item(Q, q, &slot);
+    if (it < 0) break;
+    const WorkItem w = Q.items[it];
+    const FamArgs& a = fams[w.fam];
+    const BlkDev B = a.blk[w.block];
+    const Tile& T = w.T;
+    int t = threadIdx.x;
+    int ox = (w.tile % T.ntx) * T.tx + 1, oy = ((w.tile / T.ntx) % T.nty) * T.ty + 1,
+        oz = (w.tile / (T.ntx * T.nty)) * T.tz + 1;
+    int ci = ox + t % T.tx, cj = oy + (t / T.tx) % T.ty, ck = oz + t / (T.tx * T.ty);
+    if (t < T.tx * T.ty * T.tz && ci <= B.ni && cj <= B.nj && ck <= B.nk) {
+      long long g = gidx(B, ci, cj, ck), sj = B.NI, sk = (long long)B.NI * B.NJ;
+      const double* r = a.cur;
+      double rc = r[g];
+      double ri0 = (B.flat & 1) ? rc : r[g - 1], ri1 = (B.flat & 1) ? rc : r[g + 1];
+      double rj0 = (B.flat & 2) ? rc : r[g - sj], rj1 = (B.flat & 2) ? rc : r[g + sj];
+      double rk0 = (B.flat & 4) ? rc : r[g - sk], rk1 = (B.flat & 4) ? rc : r[g + sk];
+      a.beta[g] = sd_weight(rc, ri0, ri1, rj0, rj1, rk0, rk1);
+    }
+  }
+}
+
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_dt_persist(const FamArgs* fams, WorkQueues Q) {
+  __shared__ int slot;
+  int q = blockIdx.x % Q.nq;
+  double dmin = 1.79769313486231571e308;
+  DevStatus* st = fams[0].st;
+  for (;;) {
+    int it = next_item(Q, q, &slot);
+    if (it < 0) break;
+    const WorkItem w = Q.items[it];
+    const FamArgs& a = fams[w.fam];
+    const BlkDev B = a.blk[w.block];
+    const Tile& T = w.T;
+    int t = threadIdx.x;
+    int ox = (w.tile % T.ntx) * T.tx + 1, oy = ((w.tile / T.ntx) % T.nty) * T.ty + 1,
+        oz = (w.tile / (T.ntx * T.nty)) * T.tz + 1;
+    int ci = ox + t % T.tx, cj = oy + (t / T.tx) % T.ty, ck = oz + t / (T.tx * T.ty);
+    if (t < T.tx * T.ty * T.tz && ci <= B.ni && cj <= B.nj && ck <= B.nk) dmin = dmin2(dmin, dt_cell<M>(a, gidx(B, ci, cj, ck)));
+  }
+  block_min_atomic(dmin, &st->dtbits);
+}
+
+namespace {  // one copy per translation unit
+__global__ void k_zero_tickets(unsigned long long* t, int n) {
+  if (threadIdx.x < n) t[threadIdx.x] = 0ull;
+}
+}  // namespace
 
 }  // namespace ice

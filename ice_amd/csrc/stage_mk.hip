@@ -8,4 +8,11 @@ void launch_stage_mk(const StageLaunch& L, const FamArgs& a) {
   else launch_sd_cp<0, 2>(L, a);
 }
 void launch_dt_mk(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<0>(s, blk, nb, a); }
+void launch_batched_mk(const BatchLaunch& B) {
+  if (B.riemann == 0) launch_batched_sd_cp<0, 0>(B);
+  else if (B.riemann == 1) launch_batched_sd_cp<0, 1>(B);
+  else launch_batched_sd_cp<0, 2>(B);
+}
+void launch_batched_dt_mk(const BatchLaunch& B) { launch_batched_dt_kernel<0>(B); }
+int persistent_wgs_mk(int cus) { return persistent_wgs_of<0, 1, true, true>(cus); }
 }  // namespace ice

@@ -24,9 +24,9 @@ def extrapolate(nodes: np.ndarray) -> np.ndarray:
     n[:, 1:im + 2, 1:jm + 2, 1:km + 2] = nodes
     o = 1  # offset of index 0
     # i-faces, k = 0..km, j = 0..jm
-    s = (slice(None), slice(o, jm + o + 1), slice(o, km + o + 1))
-    n[:, 0][s[1:]] = 2.0 * n[:, o][s[1:]] - n[:, o + 1][s[1:]]
-    n[:, im + o + 1][s[1:]] = 2.0 * n[:, im + o][s[1:]] - n[:, im + o - 1][s[1:]]
+    J, K = slice(o, jm + o + 1), slice(o, km + o + 1)
+    n[:, 0, J, K] = 2.0 * n[:, o, J, K] - n[:, o + 1, J, K]
+    n[:, im + o + 1, J, K] = 2.0 * n[:, im + o, J, K] - n[:, im + o - 1, J, K]
     # j-faces, k = 0..km, i = -1..im+1
     n[:, :, 0, o:km + o + 1] = 2.0 * n[:, :, o, o:km + o + 1] - n[:, :, o + 1, o:km + o + 1]
     n[:, :, jm + o + 1, o:km + o + 1] = 2.0 * n[:, :, jm + o, o:km + o + 1] - n[:, :, jm + o - 1, o:km + o + 1]

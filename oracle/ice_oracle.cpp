@@ -882,7 +882,7 @@ int orc_create(int /*device*/, const ice_config_t* cfg, const ice_family_t* fams
     B.vol.assign(B.npad, nan); B.dl.assign(B.npad * 3, nan); B.M.assign(B.npad * 9, nan);
     for (int d = 0; d < 3; ++d) B.fa[d].assign(B.npad * 4, nan);
     B.g.assign(B.npad * 5, nan); B.gR.assign(B.npad, nan); B.gG.assign(B.npad, nan);
-    B.gK.assign(B.npad, nan); B.gMu.assign(B.npad, nan);
+    B.gK.assign(B.npad, nan); B.gMu.assign(B.npad, nan); B.yinf.assign(B.npad, nan);
     for (int p = 0; p < cfg->ngroups; ++p) {
       Phase ph;
       int nc = c->fam[p].nc;
@@ -996,6 +996,15 @@ int orc_put_gas(orc_ctx* c, int b, const double* gprim, const double* R, const d
         for (int m = 0; m < 5; ++m) B.g[id * 5 + m] = gprim[s * 5 + m];
         B.gR[id] = R[s]; B.gG[id] = gam[s]; B.gK[id] = k[s]; B.gMu[id] = mu[s];
       }
+  return 0;
+}
+
+int orc_put_gas_yinf(orc_ctx* c, int b, const double* yinf) {
+  Blk& B = c->blk[b - 1];
+  for (int kk = 1; kk <= B.nk; ++kk)
+    for (int j = 1; j <= B.nj; ++j)
+      for (int i = 1; i <= B.ni; ++i)
+        B.yinf[B.id(i, j, kk)] = yinf[(size_t)(i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (kk - 1))];
   return 0;
 }
 

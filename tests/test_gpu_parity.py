@@ -57,3 +57,62 @@ def test_tiled_equals_untiled(gpu):
         ra.step(10)
         rb.step(10)
         assert np.array_equal(ra.state(1, 1), rb.state(1, 1))
+
+
+def test_multiblock_maps_and_state(gpu, oracle):
+    """Connectivity maps bit for bit, then the state of every block and family."""
+    case = cases.case_multiblock(nblocks=10, emin=6, emax=14, steps=20)
+    print("case multiblock 10 blocks", flush=True)
+    rg = cases.Runner(gpu, case, "ice_gpu_")
+    ro = cases.Runner(oracle, case, "orc_")
+    for p in range(1, case.cfg.ngroups + 1):
+        for a, b in zip(rg.bc_maps(p), ro.bc_maps(p)):
+            assert np.array_equal(a, b)
+    rg.step(20)
+    ro.step(20)
+    err = compare_states(rg, ro, case)
+    print("rel L2", err)
+    assert err <= 1e-12
+
+
+def test_multiblock_uncoupled_bit_exact(gpu, oracle):
+    case = cases.case_multiblock(nblocks=10, emin=6, emax=14, steps=20)
+    case.cfg.coupled = 0
+    rg, ro, sg, so = run_both(gpu, oracle, case, 20)
+    for b in range(1, len(case.blocks) + 1):
+        for p in range(1, case.cfg.ngroups + 1):
+            assert np.array_equal(rg.state(b, p), ro.state(b, p)), (b, p)
+
+
+@pytest.mark.parametrize("which", ["multiblock", "jets"])
+def test_batched_graph_equals_plain(gpu, which):
+    """The batched path (persistent kernels over per-XCD tile queues, the step
+    replayed from a hipGraph) gives the plain path's bits, replay after replay,
+    with host reads of the state between replays."""
+    mk = (lambda mode: cases.case_multiblock(nblocks=12, emin=6, emax=16, steps=6, mode=mode)) if which == "multiblock" \
+        else (lambda mode: cases.case_crossing_jets(n=80, steps=6, mode=mode))
+    plain, batched = mk(0), mk(1)
+    print("case", which, "batched vs plain", flush=True)
+    rp = cases.Runner(gpu, plain, "ice_gpu_")
+    rb = cases.Runner(gpu, batched, "ice_gpu_")
+    for step in range(6):
+        sp = rp.step(1)
+        sb = rb.step(1)
+        assert sp.dtglobal == sb.dtglobal and sp.iter == sb.iter
+        for b in range(1, len(plain.blocks) + 1):
+            for p in range(1, plain.cfg.ngroups + 1):
+                assert np.array_equal(rp.state(b, p), rb.state(b, p)), (step, b, p)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_residual_norm(gpu, oracle, mode):
+    """residual_norm of the last step against the oracle's (fixed-order tree sums
+    on the device, sequential sums on the host: 1e-12 relative)."""
+    case = cases.case_multiblock(nblocks=10, emin=6, emax=14, steps=5, mode=mode)
+    print("case multiblock residual_norm mode", mode, flush=True)
+    rg, ro, sg, so = run_both(gpu, oracle, case, 5)
+    tg, fg = rg.residual_norm()
+    to, fo = ro.residual_norm()
+    assert np.all(to > 0)
+    assert np.allclose(tg, to, rtol=1e-12, atol=0)
+    assert np.allclose(fg, fo, rtol=1e-12, atol=0)
