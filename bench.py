@@ -44,6 +44,7 @@ WORKLOADS = {
     "jets2d": lambda mode: cases.case_crossing_jets(n=1024, steps=500, mode=mode),
     "riemann1d": lambda mode: cases.case_riemann_1d(n=2000, steps=1000, mode=mode),
     "multiblock3d": lambda mode: cases.case_multiblock(mode=mode),
+    "evap3d": lambda mode: cases.case_evaporation(n=256, steps=50, mode=mode),
 }
 DESCR = {
     "jets2d": "2-D crossing jets, 1024x1024 cells, AG closure, HLLE, MUSCL van Leer + Jameson sensor, "
@@ -51,6 +52,9 @@ DESCR = {
     "riemann1d": "1-D IG Riemann problem, 2000 cells, MUSCL minmod, Rusanov, drag, SSP-RK3, FP64",
     "multiblock3d": "3-D multi-block tree, 64 blocks (edges log-uniform 16-128, 6.26M cells), 3 IG families, "
                     "Rusanov, MUSCL van Leer, drag, SSP-RK3, FP64",
+    "evap3d": "3-D evaporation-heavy block, 256^3 cells, MK closure, Saurel, MUSCL van Leer, drag Schiller-Naumann + "
+              "Ranz-Marshall heat + grey-body radiation + CEM evaporation with Langmuir-Knudsen interface and Stefan "
+              "blowing, per-cell Yinf 0-0.3, gas 300-2000 K / 1-5 bar, SSP-RK3, FP64",
 }
 
 

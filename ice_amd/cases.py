@@ -516,3 +516,69 @@ def case_multiblock(nblocks: int = 64, emin: int = 16, emax: int = 128, seed: in
     fams = [make_family(IG, RUSANOV, rho_mat=rho_mat, cs=4186.0, lv=2.26e6) for _ in range(ngroups)]
     return Case("multiblock", cfg, fams, out, reci, recd, donor, weight, steps,
                 meta={"joins": joins, "origins": orig})
+
+
+# ---------------------------------------------------------------- case 4
+def water_ep(yinf: float = 0.0, alphae: float = 0.04, cpv: float = 0.0):
+    """ep(1:8) of water vapour as Setup_Materials packs it (Mv, Lv, cpv, Le, Yinf,
+    Lv Mv / Ru, 1/Tboil, alphaE), values of test/verification/F-evaporation/run.py"""
+    Mv, Lv, Tb = 18.015, 2.26e6, 373.15
+    return [Mv, Lv, cpv, 1.0, yinf, Lv * Mv / 8314.46, 1.0 / Tb, alphae]
+
+
+def case_evaporation(n: int = 256, steps: int = 50, seed: int = 20261206, mode: int = 0, evap: int = 2,
+                     intf: int = 1, blow: int = 1, box: float = 25.6e-3, limiter: int = 3) -> Case:
+    """Evaporation-heavy: one n^3 block of side `box` (25.6 mm at n = 256), MK closure
+    (Saurel flux), water drops in a frozen gas with seeded smooth random fields
+    (16 modes each): T 300-2000 K, p 1-5 bar (rho_g = p / (R T)), far-field vapour
+    mass fraction 0-0.3 per cell, velocity |u| <= 10 m/s.  Drops: radius lognormal
+    (median 20 um, sigma 0.6, smooth), clipped to 1-100 um; T_d 280-350 K; rho_p =
+    1e-2 (1 + 0.3 noise); at rest.  Drag Schiller-Naumann, Ranz-Marshall heat,
+    grey-body radiation (emissivity 0.9), evaporation `evap` (2 CEM, 4 ASM) with
+    the Langmuir-Knudsen interface (intf 1) and Stefan blowing (blow 1).  SSP-RK3,
+    time-accurate, transmissive (400) faces."""
+    rng = np.random.default_rng(seed)
+    dims = (n, n, n)
+    dx = box / n
+    geo = geometry.uniform_metrics(dims, dx)
+    c = (np.arange(n) + 0.5) / n
+
+    def unit(f):
+        return (f - f.min()) / (f.max() - f.min())
+
+    def field():
+        return geometry.smooth_field_axes(rng, c, c, c)
+
+    u, v, w = field(), field(), field()
+    umax = np.sqrt(u * u + v * v + w * w).max()
+    Tg = 300.0 + 1700.0 * unit(field())
+    p = 1e5 + 4e5 * unit(field())
+    yinf = 0.3 * unit(field())
+    zr = field()
+    r = np.clip(20e-6 * np.exp(0.6 * zr / zr.std()), 1e-6, 100e-6)
+    Td = 280.0 + 70.0 * unit(field())
+    noise = field()
+    rho = 1e-2 * (1.0 + 0.3 * noise / np.abs(noise).max())
+    R, gam = 287.05, 1.4
+    g5 = np.stack([p / (R * Tg), 10.0 * u / umax, 10.0 * v / umax, 10.0 * w / umax, Tg])
+    mu = 1.8e-5 * (Tg / 300.0) ** 0.7
+    kg = 0.026 * (Tg / 300.0) ** 0.7
+    gas = gas_arrays(g5, R, gam, kg, mu)
+    rho_mat = 1000.0
+    num = rho / (rho_mat * 4.0 / 3.0 * PI * r ** 3)
+
+    def fill(I, J, K):
+        z = np.zeros(I.shape)
+        return np.stack([rho, z, z, z, Td, num])
+
+    prim = padded_state(dims, 6, fill)
+    blk = Block(dims, geo, [prim], gas, yinf=np.ascontiguousarray(yinf.ravel(order="F")))
+    tb = TableBuilder(1)
+    tb.add_block(1, dims, lambda f, i, j, k: np.full(len(f), 400, np.int32))
+    reci, recd, donor, weight = tb.arrays()
+    cfg = make_config(ngroups=1, nrk=3, time_accurate=True, ndir=3, limiter=limiter, coupled=True,
+                      drag_select=4, heat_select=6, blow_select=blow, var_yinf=True, cfl=0.5, dt_max=2e-6,
+                      tau_factor=0.5, mode=mode)
+    fam = make_family(MK, SAUREL, rho_mat=rho_mat, cs=4180.0, lv=2.26e6, emiss=0.9, evap=evap, intf=intf,
+                      ep=water_ep())
+    return Case("evaporation", cfg, [fam], [blk], reci, recd, donor, weight, steps)

@@ -6,6 +6,7 @@
 // Geometry and gas use the same index, so every stream is unit-stride in i.
 #pragma once
 #include "dev_physics.hpp"
+#include "dev_evap.hpp"
 
 namespace ice {
 
@@ -128,7 +129,9 @@ ICE_DEV bool face_acc(const FamArgs& a, long long g, long long stride, int d, in
 
 // ------------------------------------------------------------ cell source
 // compute_source_ (Mod_Sources.f90:465-622) with cell_surface (:230-274), the
-// ITC surface (Lib_Drop_Surface.f90:15-43), no combustion, no P2T/boiling.
+// ITC surface (Lib_Drop_Surface.f90:15-43), evaporation with the interface and
+// blowing models (Lib_Evaporation.f90), no combustion, no P2T/boiling.  With a
+// per-cell far-field vapour fraction its value replaces ep(iYinf).
 template <int M>
 ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double* src, double& tau) {
   constexpr int NC = Closure<M>::NC;
@@ -153,7 +156,18 @@ ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double*
   F[2] = gp[2] - q[2];
   F[3] = gp[3] - q[3];
   double Qconv = 2.0 * Nu * gk * kPi * Rp * (gp[4] - Ts);
-  double mdot = 0.0;
+  double mdot = 0.0, Qevap = 0.0;
+  bool ovr = false;
+  if (mt.evap != 0) {
+    double ep[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) ep[m] = mt.ep[m];
+    if (a.var_yinf) ep[kYinf] = a.yinf[g];
+    evaporation_d(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, mt.evap, mt.intf, ep, mdot, Qevap, ovr);
+  }
+  if (ovr) Qconv = Qevap;
+  else if (a.blow_sel == 1)
+    Qconv = Qconv * blowing_factor_d(gam, R, mu, gk, rho_mat, 2.0 * Rp, 4.0 / 3.0 * kPi * (Rp * Rp * Rp) * rho_mat, mdot);
   F[0] = -mdot * q[NC - 1];
   F[4] = Qconv * q[NC - 1];
   F[4] = F[4] + mt.emiss * kSigmaSB * 4.0 * kPi * Rp * Rp * q[NC - 1] * (pow(gp[4], 4.0) - pow(Ts, 4.0));

@@ -157,3 +157,42 @@ def cartesian_nodes(x: np.ndarray, y: np.ndarray, z: np.ndarray) -> np.ndarray:
     """(3, nx, ny, nz) node coordinates of a Cartesian block from its node lines."""
     X, Y, Z = np.meshgrid(np.asarray(x, float), np.asarray(y, float), np.asarray(z, float), indexing="ij")
     return np.stack([X, Y, Z])
+
+
+def uniform_metrics(dims, dx: float) -> dict:
+    """The metrics() of a uniform Cartesian block of cubes of side dx, in closed form
+    (equal to metrics() on such a grid to rounding), for blocks too large for the
+    node-based construction: vol = dx^3, lengths dx, M = I/dx, unit normals along
+    the axes with area dx^2."""
+    ni, nj, nk = dims
+    f = lambda a: np.ascontiguousarray(a.ravel(order="F"))  # noqa: E731
+    vol = np.full((ni, nj, nk), dx * dx * dx)
+    dl = np.full((3, ni + 2, nj + 2, nk + 2), dx)
+    M = np.zeros((3, 3, ni, nj, nk))
+    for d in range(3):
+        M[d, d] = 1.0 / dx
+
+    def face(d, shape):
+        a = np.zeros((4,) + shape)
+        a[d] = 1.0
+        a[3] = dx * dx
+        return a
+
+    return {"dims": (ni, nj, nk), "vol": f(vol), "dl": f(dl), "M": f(M),
+            "face_i": f(face(0, (ni + 1, nj, nk))), "face_j": f(face(1, (ni, nj + 1, nk))),
+            "face_k": f(face(2, (ni, nj, nk + 1)))}
+
+
+def smooth_field_axes(rng, x, y, z, modes: int = 16, kmax: int = 4):
+    """A sum of `modes` random sine modes (wave numbers 1..kmax per axis) on the
+    tensor grid of the axes x, y, z, by the angle-addition form of the sine: the
+    same draws as a pointwise evaluation, one complex product per cell and mode."""
+    f = np.zeros((len(x), len(y), len(z)))
+    for _ in range(modes):
+        k = rng.integers(1, kmax + 1, size=3) * rng.choice([-1, 1], size=3)
+        amp, ph = rng.normal(), rng.uniform(0, 2 * np.pi)
+        ex = np.exp(1j * (2 * np.pi * k[0] * np.asarray(x) + ph))
+        ey = np.exp(1j * 2 * np.pi * k[1] * np.asarray(y))
+        ez = np.exp(1j * 2 * np.pi * k[2] * np.asarray(z))
+        f += amp * (ex[:, None, None] * (ey[:, None] * ez[None, :])[None]).imag
+    return f

@@ -17,6 +17,7 @@
 
 #include "../include/ice_gpu.h"
 #include "oracle_physics.hpp"
+#include "oracle_evap.hpp"
 
 using namespace orc;
 
@@ -568,8 +569,12 @@ void compute_flux(orc_ctx* c, int p) {
 // ---------------------------------------------------------------- sources
 // Mod_Sources.f90:465-622 with cell_surface (:230-274), ITC surface
 // (Lib_Drop_Surface.f90:15-43) and no evaporation model
+// compute_source_ (Mod_Sources.f90:465-622) with cell_surface (:228-267) and an
+// isothermal drop (drop_surface, Lib_Drop_Surface.f90:15-43: Ts = T without P2T
+// or boiling), constant material properties and no combustion.  `yinf` replaces
+// ep(iYinf) when the far-field vapour fraction is given per cell.
 void source_cell(orc_ctx* c, const Family& F, const double* q, const double* gp, double R, double gam, double gk,
-                 double mu, double* tau, double* src) {
+                 double mu, double yinf, double* tau, double* src) {
   int n = F.nbase;
   double rho_mat = F.mat.rho;
   double Rp = std::pow(0.75 * q[0] / (q[n - 1] * PI * rho_mat), 1.0 / 3.0);
@@ -584,7 +589,15 @@ void source_cell(orc_ctx* c, const Family& F, const double* q, const double* gp,
   double force[6];
   force[1] = gp[1] - q[1]; force[2] = gp[2] - q[2]; force[3] = gp[3] - q[3];
   double Qconv = 2.0 * Nu * gk * PI * Rp * (gp[4] - Ts);
-  double mdot = 0.0;
+  double mdot = 0.0, Qevap = 0.0;
+  bool ovr = false;
+  double ep[8];
+  for (int m = 0; m < 8; ++m) ep[m] = F.mat.ep[m];
+  if (c->cfg.var_yinf) ep[EP_YINF] = yinf;
+  evaporation(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, F.mat.evap, F.mat.intf, ep, mdot, Qevap, ovr);
+  if (ovr) Qconv = Qevap;
+  else if (c->cfg.blow_select == 1)
+    Qconv = Qconv * blowing_factor(gam, R, mu, gk, rho_mat, 2.0 * Rp, 4.0 / 3.0 * PI * (Rp * Rp * Rp) * rho_mat, mdot);
   force[0] = -mdot * q[n - 1];
   force[4] = Qconv * q[n - 1];
   force[4] = force[4] + F.mat.emiss * SIGMA_SB * 4.0 * PI * Rp * Rp * q[n - 1] *
@@ -606,7 +619,8 @@ void compute_source(orc_ctx* c, int p) {
         for (int i = 1; i <= B.ni; ++i) {
           size_t id = B.id(i, j, k);
           double* s = &ph.source[id * nc];
-          source_cell(c, F, &ph.prim[id * nc], &B.g[id * 5], B.gR[id], B.gG[id], B.gK[id], B.gMu[id], &ph.tau[id], s);
+          source_cell(c, F, &ph.prim[id * nc], &B.g[id * 5], B.gR[id], B.gG[id], B.gK[id], B.gMu[id], B.yinf[id],
+                      &ph.tau[id], s);
           for (int f = 0; f < F.nfrozen; ++f) s[F.frozen[f] - 1] = 0.0;
         }
   }
@@ -1098,4 +1112,14 @@ void orc_fn_rk_stage(int irk, int nrk, int nc, const double* cu, const double* c
   }
 }
 double orc_fn_norm2(double a, double b, double c) { return norm2_3(a, b, c); }
+void orc_fn_evaporation(double rhog, double Tg, double gam, double Rg, double mug, double kg, double Tp, double dp,
+                        double Re, int evap, int intf, const double* ep, double* mdot, double* qdot, int* ovr,
+                        int* iters) {
+  bool o = false;
+  evaporation(rhog, Tg, gam, Rg, mug, kg, Tp, dp, Re, evap, intf, ep, *mdot, *qdot, o, iters);
+  *ovr = o ? 1 : 0;
+}
+double orc_fn_blowing(double gam, double Rg, double mug, double kg, double rhop, double dp, double mp, double mdot) {
+  return blowing_factor(gam, Rg, mug, kg, rhop, dp, mp, mdot);
+}
 }

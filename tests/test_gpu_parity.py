@@ -109,6 +109,7 @@ def test_residual_norm(gpu, oracle, mode):
     """residual_norm of the last step against the oracle's (fixed-order tree sums
     on the device, sequential sums on the host: 1e-12 relative)."""
     case = cases.case_multiblock(nblocks=10, emin=6, emax=14, steps=5, mode=mode)
+    case.cfg.coupled = 0  # the states are then bit-exact and only the summation order differs
     print("case multiblock residual_norm mode", mode, flush=True)
     rg, ro, sg, so = run_both(gpu, oracle, case, 5)
     tg, fg = rg.residual_norm()
@@ -116,3 +117,17 @@ def test_residual_norm(gpu, oracle, mode):
     assert np.all(to > 0)
     assert np.allclose(tg, to, rtol=1e-12, atol=0)
     assert np.allclose(fg, fo, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("evap,intf,blow", [(1, 0, 0), (2, 1, 1), (3, 1, 1), (4, 1, 0), (5, 1, 0), (2, 0, 1)])
+def test_evaporation_case(gpu, oracle, evap, intf, blow):
+    """The evaporation-heavy case (MK, per-cell far-field vapour, drag, heat,
+    radiation, evaporation with the Langmuir-Knudsen interface and blowing) at
+    20^3 cells, 8 steps: 1e-12 relative L2 per variable."""
+    case = cases.case_evaporation(n=20, steps=8, evap=evap, intf=intf, blow=blow)
+    print("case evaporation 20^3", evap, intf, blow, flush=True)
+    rg, ro, sg, so = run_both(gpu, oracle, case, 8)
+    assert sg.n_floored == so.n_floored and sg.bad_state == so.bad_state == 0
+    err = compare_states(rg, ro, case)
+    print("rel L2", err)
+    assert err <= 1e-12

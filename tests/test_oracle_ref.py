@@ -225,3 +225,56 @@ def test_rk_stage(ref, orc):
                 ref.ref_rk_stage(irk, nrk, 12, cu, co, r, a)
                 orc.orc_fn_rk_stage(irk, nrk, 12, cu, co, r, b)
                 assert same(a, b)
+
+
+def water_ep(yinf=0.0, alphae=1.0, cpv=0.0):
+    """ep(1:8) of water vapour as Setup_Materials packs it: Mv, Lv, cpv, Le, Yinf,
+    Lv Mv/Ru, 1/Tboil, alphaE (values of test/verification/F-evaporation/run.py)"""
+    Mv, Lv, Tb = 18.015, 2.26e6, 373.15
+    return np.array([Mv, Lv, cpv, 1.0, yinf, Lv * Mv / 8314.46, 1.0 / Tb, alphae])
+
+
+def test_evaporation_models(ref, orc):
+    """Every gas-side model with and without the Langmuir-Knudsen interface, over
+    the evaporation-heavy ranges (Tg 300-2000 K, 1-5 bar, Yinf 0-0.3, Tp 280-370 K,
+    d 2-200 um, Re 0-300), bit for bit against the flang-built module."""
+    ref.ref_evaporation.argtypes = [D] * 9 + [C.c_int, C.c_int, P, C.POINTER(D), C.POINTER(D),
+                                              C.POINTER(C.c_int)]
+    orc.orc_fn_evaporation.argtypes = [D] * 9 + [C.c_int, C.c_int, P, C.POINTER(D), C.POINTER(D),
+                                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    rng = np.random.default_rng(900)
+    lk_iters = []
+    for t in range(2500):
+        Tg, p = rng.uniform(300, 2000), rng.uniform(1e5, 5e5)
+        R, gam, mu, k = 287.05, 1.4, 1.8e-5 * (Tg / 300) ** 0.7, 0.026 * (Tg / 300) ** 0.7
+        rhog = p / (R * Tg)
+        Tp, dp = rng.uniform(280, 370), 10 ** rng.uniform(np.log10(2e-6), np.log10(2e-4))
+        Re = 0.0 if t % 10 == 0 else 10 ** rng.uniform(-3, 2.5)
+        ep = water_ep(yinf=rng.uniform(0, 0.3), alphae=10 ** rng.uniform(-2, 0),
+                      cpv=0.0 if t % 2 else 1800.0)
+        for evap in range(0, 6):
+            for intf in (0, 1):
+                m1, q1, o1 = D(), D(), C.c_int()
+                m2, q2, o2, it = D(), D(), C.c_int(), C.c_int()
+                ref.ref_evaporation(rhog, Tg, gam, R, mu, k, Tp, dp, Re, evap, intf, ep, C.byref(m1), C.byref(q1),
+                                    C.byref(o1))
+                orc.orc_fn_evaporation(rhog, Tg, gam, R, mu, k, Tp, dp, Re, evap, intf, ep, C.byref(m2),
+                                       C.byref(q2), C.byref(o2), C.byref(it))
+                assert (m1.value, q1.value, o1.value) == (m2.value, q2.value, o2.value), (t, evap, intf)
+                if intf == 1 and m1.value < 0:
+                    lk_iters.append(it.value)
+    assert lk_iters and max(lk_iters) > 2  # the Picard loop is exercised
+
+
+def test_blowing_factor(ref, orc):
+    ref.ref_blowing.argtypes = [D] * 8
+    ref.ref_blowing.restype = D
+    orc.orc_fn_blowing.argtypes = [D] * 8
+    orc.orc_fn_blowing.restype = D
+    rng = np.random.default_rng(901)
+    for t in range(5000):
+        dp = 10 ** rng.uniform(-6, -3.7)
+        mp = 1000.0 * np.pi / 6 * dp ** 3
+        mdot = -(10 ** rng.uniform(-14, -5)) if t % 7 else rng.uniform(0, 1e-9)
+        args = (rng.uniform(1.1, 1.7), 287.05, 1.8e-5, 0.026, 1000.0, dp, mp, mdot)
+        assert ref.ref_blowing(*args) == orc.orc_fn_blowing(*args), args
