@@ -582,3 +582,99 @@ def case_evaporation(n: int = 256, steps: int = 50, seed: int = 20261206, mode: 
     fam = make_family(MK, SAUREL, rho_mat=rho_mat, cs=4180.0, lv=2.26e6, emiss=0.9, evap=evap, intf=intf,
                       ep=water_ep())
     return Case("evaporation", cfg, [fam], [blk], reci, recd, donor, weight, steps)
+
+
+# ---------------------------------------------------------------- chimera (overset) pair
+def case_chimera(n: int = 12, na: int = 16, shift: float = 3.7, closure: int = IG, ngroups: int = 1,
+                 irs: bool = False, evap: int = 0, steps: int = 10, seed: int = 20261207, mode: int = 0,
+                 limiter: int = 3, connect: bool = False) -> Case:
+    """Two overlapping blocks joined by chimera (102) records: A is na x n x n cells,
+    B (na x n x n) starts `shift` cells before A's upper i-end, so that the two
+    ghost layers of B's face 1 lie inside A and those of A's face 2 inside B.  Each
+    ghost takes the conservative state of the two A (or B) cells bracketing its
+    centre along i, with linear weights summing to 1 per layer (the donor lists of
+    IO_BC.f90:209-235).  Other faces extrapolate (400).  A smooth particle field
+    moves with +x velocity through the overlap; the gas is a smooth random field.
+    `closure` IG or AG, `ngroups` families, IRS (steady) and evaporation on demand.
+    With shift = 0 the blocks abut; `connect` then joins them by face-matching
+    connections (101) instead, the same interface by another route."""
+    rng = np.random.default_rng(seed)
+    dx = 1e-3
+    dims = (na, n, n)
+    x0 = [0.0, (na - shift) * dx]
+    gseed = int(rng.integers(1 << 30))
+    blocks = []
+    rho_mat = 1000.0
+    nc = NCOND[closure]
+    ext = (x0[1] + na * dx)
+    for b in range(2):
+        xs = [x0[b] + np.arange(na + 1) * dx, np.arange(n + 1) * dx, np.arange(n + 1) * dx]
+        geo = geometry.metrics(geometry.cartesian_nodes(*xs))
+        xc = [x0[b] + (np.arange(na) + 0.5) * dx, (np.arange(n) + 0.5) * dx, (np.arange(n) + 0.5) * dx]
+        X, Y, Z = np.meshgrid(xc[0] / ext, xc[1] / (n * dx), xc[2] / (n * dx), indexing="ij")
+        g_rng = np.random.default_rng(gseed)
+        u, v, w, t = (smooth_field(g_rng, X, Y, Z, modes=6, kmax=2) for _ in range(4))
+        g5 = np.stack([np.full(X.shape, 1.2), 8.0 + u, v, w, 900.0 + 100.0 * t])
+        gas = gas_arrays(g5, 287.05, 1.4, 0.026, 1.8e-5)
+        prims = []
+        for p in range(ngroups):
+            r0 = [20e-6, 50e-6][p % 2]
+
+            def fill(I, J, K, X=X, Y=Y, Z=Z, r0=r0):
+                rho = 1e-3 * (1.0 + 0.3 * np.sin(2 * PI * X) * np.cos(2 * PI * Y) * np.cos(2 * PI * Z))
+                num = rho / (rho_mat * 4.0 / 3.0 * PI * r0 ** 3)
+                z = np.zeros(I.shape)
+                uu, vv = 5.0 + 0.5 * np.sin(2 * PI * Y), 0.5 * np.cos(2 * PI * X)
+                if closure == AG:
+                    th = rho * 1.0
+                    return np.stack([rho, uu, vv, z, th, z, z, th, z, th, z + 330.0, num])
+                return np.stack([rho, uu, vv, z, rho * 1.0, z + 330.0, num])
+
+            prims.append(padded_state(dims, nc, fill))
+        blocks.append(Block(dims, geo, prims, gas,
+                            yinf=np.zeros(dims[0] * dims[1] * dims[2]) if evap else None))
+    tb = TableBuilder(ngroups)
+
+    def donors(other, xg):
+        """(cells, weights) of block `other` bracketing centre x = xg along i"""
+        s = (xg - x0[other]) / dx + 0.5  # fractional 1-based index
+        i0 = int(np.floor(s + 1e-9))
+        f = s - i0
+        if abs(f) < 1e-9:
+            return [(i0, 1.0)]
+        return [(i0, 1.0 - f), (i0 + 1, f)]
+
+    for b in range(2):
+        other = 1 - b
+        face = 1 if b == 1 else 2
+        for (f, i, j, k) in face_cell_array(dims):
+            if f != face:
+                tb.add(b + 1, int(f), int(i), int(j), int(k), 400)
+                continue
+            if connect:
+                # the same interface as a face-matching connection (shift 0)
+                src = 1 if face == 2 else na
+                tb.add(b + 1, int(f), int(i), int(j), int(k), 101,
+                       ints={R_BS: other + 1, R_IS: src, R_JS: int(j), R_KS: int(k), R_FS: 3 - face})
+                continue
+            d0 = len(tb.weight)
+            nchi = []
+            for g in (1, 2):
+                ig = i - g if face == 1 else i + g
+                xg = x0[b] + (ig - 0.5) * dx
+                dd = donors(other, xg)
+                nchi.append(len(dd))
+                for (io, wgt) in dd:
+                    assert 1 <= io <= na
+                    tb.donor.extend([other + 1, io, int(j), int(k)])
+                    tb.weight.append(wgt)
+            tb.add(b + 1, int(f), int(i), int(j), int(k), 102,
+                   ints={R_NCHI1: nchi[0], R_NCHI2: nchi[1], R_DONOR0: d0})
+    reci, recd, donor, weight = tb.arrays()
+    cfg = make_config(ngroups=ngroups, nrk=3, time_accurate=not irs, ndir=3, limiter=limiter, coupled=True,
+                      drag_select=4, heat_select=6, blow_select=1 if evap else 0, var_yinf=bool(evap), irs=irs,
+                      cfl=0.5, dt_max=1e-4, tau_factor=0.5, mode=mode)
+    fams = [make_family(closure, HLLE if closure == AG else RUSANOV, rho_mat=rho_mat, cs=4180.0, lv=2.26e6,
+                        emiss=0.9 if evap else 0.0, evap=evap, intf=1 if evap else 0, ep=water_ep(alphae=0.04))
+            for _ in range(ngroups)]
+    return Case("chimera", cfg, fams, blocks, reci, recd, donor, weight, steps, meta={"x0": x0})

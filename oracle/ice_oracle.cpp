@@ -1112,6 +1112,25 @@ void orc_fn_rk_stage(int irk, int nrk, int nc, const double* cu, const double* c
   }
 }
 double orc_fn_norm2(double a, double b, double c) { return norm2_3(a, b, c); }
+// residual_smoothing of one block (Lib_IRS.f90:15-31): directions 1..ndir in
+// turn on res(nc, ni, nj, nk), interior only, variable fastest
+void orc_fn_irs(int ni, int nj, int nk, int nc, int ndir, double beta, double* res) {
+  Blk B;
+  B.ni = ni; B.nj = nj; B.nk = nk; B.NI = ni + 4; B.NJ = nj + 4; B.NK = nk + 4;
+  B.npad = (size_t)B.NI * B.NJ * B.NK;
+  std::vector<double> r(B.npad * nc, 0.0);
+  for (int k = 1; k <= nk; ++k)
+    for (int j = 1; j <= nj; ++j)
+      for (int i = 1; i <= ni; ++i)
+        for (int v = 0; v < nc; ++v)
+          r[B.id(i, j, k) * nc + v] = res[(((size_t)(k - 1) * nj + (j - 1)) * ni + (i - 1)) * nc + v];
+  for (int d = 0; d < ndir; ++d) irs_block(B, nc, d, beta, r);
+  for (int k = 1; k <= nk; ++k)
+    for (int j = 1; j <= nj; ++j)
+      for (int i = 1; i <= ni; ++i)
+        for (int v = 0; v < nc; ++v)
+          res[(((size_t)(k - 1) * nj + (j - 1)) * ni + (i - 1)) * nc + v] = r[B.id(i, j, k) * nc + v];
+}
 void orc_fn_evaporation(double rhog, double Tg, double gam, double Rg, double mug, double kg, double Tp, double dp,
                         double Re, int evap, int intf, const double* ep, double* mdot, double* qdot, int* ovr,
                         int* iters) {

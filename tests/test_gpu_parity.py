@@ -131,3 +131,32 @@ def test_evaporation_case(gpu, oracle, evap, intf, blow):
     err = compare_states(rg, ro, case)
     print("rel L2", err)
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(closure=2, ngroups=2), dict(closure=2, ngroups=2, irs=True, evap=2),
+                                dict(irs=True)], ids=["ig", "ag2", "ag2-irs-evap", "ig-irs"])
+def test_chimera_pair(gpu, oracle, kw):
+    """Two overset blocks with chimera ghosts (two donors per layer), optionally
+    the combined physics of the steady multi-family case: AG, two families, IRS,
+    evaporation: 1e-12 relative L2 per variable after 10 steps."""
+    case = cases.case_chimera(steps=10, **kw)
+    print("case chimera", kw, flush=True)
+    rg, ro, sg, so = run_both(gpu, oracle, case, 10)
+    assert sg.bad_state == so.bad_state == 0
+    err = compare_states(rg, ro, case)
+    print("rel L2", err)
+    assert err <= 1e-12
+
+
+def test_multiblock_irs_steady(gpu, oracle):
+    """Steady run (per-cell dt) with implicit residual smoothing over 10 blocks,
+    uncoupled: bit for bit."""
+    case = cases.case_multiblock(nblocks=10, emin=6, emax=14, steps=10)
+    case.cfg.coupled = 0
+    case.cfg.irs = 1
+    case.cfg.time_accurate = 0
+    print("case multiblock irs steady", flush=True)
+    rg, ro, sg, so = run_both(gpu, oracle, case, 10)
+    for b in range(1, len(case.blocks) + 1):
+        for p in range(1, case.cfg.ngroups + 1):
+            assert np.array_equal(rg.state(b, p), ro.state(b, p)), (b, p)
