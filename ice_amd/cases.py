@@ -678,3 +678,63 @@ def case_chimera(n: int = 12, na: int = 16, shift: float = 3.7, closure: int = I
                         emiss=0.9 if evap else 0.0, evap=evap, intf=1 if evap else 0, ep=water_ep(alphae=0.04))
             for _ in range(ngroups)]
     return Case("chimera", cfg, fams, blocks, reci, recd, donor, weight, steps, meta={"x0": x0})
+
+
+# ---------------------------------------------------------------- cases from ICE's input files
+_LIMITER_WORDS = {"none": 0, "minmod": 1, "vanalbada": 2, "vanleer": 3, "ospre": 4, "umist": 5, "osher": 6,
+                  "sweby": 7, "mc": 8, "koren": 9, "superbee": 10}
+_CLOSURE_WORDS = {"MK": MK, "IG": IG, "AG": AG}
+_RIEMANN_WORDS = {"saurel": SAUREL, "rusanov": RUSANOV, "hlle": HLLE}
+
+
+def _ini_bool(v: str) -> bool:
+    return v.strip().lower() in (".true.", "true", "t", "1", "yes")
+
+
+def config_from_ini(ini: dict, ndir: int, mode: int = 0):
+    """ice_config_t and the families of an uncoupled run from the INI sections
+    [ICE-Numerics] and [ICE-FamilyN] (defaults of Register_Numerics.f90:28-80)."""
+    num = ini.get("ICE-Numerics", {})
+    phys = ini.get("ICE-Physics", {})
+    if phys.get("drag", "NoDrag") != "NoDrag" or phys.get("heat-transfer", "NoHeat") != "NoHeat":
+        raise ValueError("a coupled run needs its gas fields: build it with the case builders")
+    scheme = num.get("time-scheme", "RK2")
+    nrk = {"euler": 1, "RK2": 2, "RK3": 3}[scheme]
+    recon = num.get("space-reconstruction", "first-order")
+    lim = _LIMITER_WORDS[num.get("flux-limiter", "none").lower()] if recon == "MUSCL" else 0
+    fams = []
+    p = 1
+    while f"ICE-Family{p}" in ini:
+        fams.append(_CLOSURE_WORDS[ini[f"ICE-Family{p}"].get("closure", "IG")])
+        p += 1
+    cfg = make_config(ngroups=len(fams), nrk=nrk, time_accurate=_ini_bool(num.get("time-accurate", ".true.")),
+                      cfl_ramp_iter=int(num.get("cfl-rise-threshold", "0")), ndir=ndir, limiter=lim,
+                      shock_detector=num.get("shock-detector", "none").lower() == "jameson",
+                      irs=_ini_bool(num.get("irs", ".false.")), coupled=False, mode=mode,
+                      cfl=float(num.get("cfl", "0.5")), dt_max=float(num.get("dt-max", "1e-4")),
+                      tau_factor=float(num.get("tau-factor", "1.0")), irs_beta=float(num.get("irs-beta", "0.5")))
+    rs = num.get("riemann-solver", "").strip().lower()
+    return cfg, [(c, _RIEMANN_WORDS[rs] if rs else None) for c in fams]
+
+
+def case_from_inputs(name: str, ini: dict, nodes: List[np.ndarray], ic: List[np.ndarray], reci, recd, donor, weight,
+                     rho_mat: float, cs: float, steps: int = 1, mode: int = 0) -> Case:
+    """A Case from what ICE's set-up reads: the INI sections, each block's nodes
+    (3, ni+1, nj+1, nk+1), its initial primitive cells (nc, ni, nj, nk) of the one
+    family, the boundary table, and the material's density and heat capacity."""
+    geos = [geometry.metrics(n) for n in nodes]
+    dims = [g["dims"] for g in geos]
+    ndir = sum(1 for d in range(3) if max(dm[d] for dm in dims) > 1)
+    cfg, fspec = config_from_ini(ini, ndir, mode)
+    if len(fspec) != 1:
+        raise ValueError("the initial condition holds one family")
+    closure, riemann = fspec[0]
+    fam = make_family(closure, riemann, rho_mat=rho_mat, cs=cs)
+    nc = NCOND[closure]
+    blocks = []
+    for g, c in zip(geos, ic):
+        if c.shape[0] != nc:
+            raise ValueError(f"initial condition with {c.shape[0]} variables for a closure of {nc}")
+        prim = padded_state(g["dims"], nc, lambda I, J, K, c=c: c[:, I - 1, J - 1, K - 1])
+        blocks.append(Block(g["dims"], g, [prim]))
+    return Case(name, cfg, [fam], blocks, reci, recd, donor, weight, steps)
