@@ -232,3 +232,37 @@ def read_ini(path: str) -> Dict[str, Dict[str, str]]:
             k, v = line.split("=", 1)
             out[sec][k.strip()] = v.strip()
     return out
+
+
+def write_bc(path: str, reci: np.ndarray, recd: np.ndarray, donor: np.ndarray, weight: np.ndarray,
+             ngroups: int = 1) -> None:
+    """An ATLAS bc.txt with one copy for all families from a boundary table in the
+    C-ABI layout (the inverse of read_bc for family 1's records)."""
+    with open(path, "w") as fh:
+        for n in range(0, reci.shape[0], ngroups):
+            r, d = reci[n], recd[n]
+            code = int(r[R_TYPE])
+            fh.write(f"{r[R_B]:8d}{r[R_I]:8d}{r[R_J]:8d}{r[R_K]:8d}{r[R_F]:8d}{code:8d}\n")
+            if code in (101, 201):
+                fh.write(f"{r[R_BS]:8d}{r[R_IS]:8d}{r[R_JS]:8d}{r[R_KS]:8d}{r[R_FS]:8d}       1       0       0       1\n")
+            elif code == 102:
+                n1, n2, d0 = int(r[R_NCHI1]), int(r[R_NCHI2]), int(r[R_DONOR0])
+                fh.write(f"{n1:8d}{n2:8d}\n")
+                for s in range(d0, d0 + n1 + n2):
+                    fh.write(f"{donor[s, 0]:8d}{donor[s, 1]:8d}{donor[s, 2]:8d}{donor[s, 3]:8d} {weight[s]:.17E}\n")
+            elif 401 <= code <= 403:
+                ang = ("normal", "normal") if r[R_NORMAL_DIR] else (f"{d[RD_ALPHA]:.17E}", f"{d[RD_BETA]:.17E}")
+                line = (f" {d[RD_MASSFLUX]:.17E} {d[RD_VELOCITY]:.17E} {ang[0]} {ang[1]} {d[RD_TEMPERATURE]:.17E} "
+                        f"{d[RD_RADIUS]:.17E} 0.0 Dirac 0.0")
+                if code == 401:
+                    line += f" {d[RD_GFIXED]:.17E}"
+                fh.write(line + "\n")
+
+
+def write_ini(path: str, ini: Dict[str, Dict[str, str]]) -> None:
+    with open(path, "w") as fh:
+        for sec, kv in ini.items():
+            fh.write(f"[{sec}]\n")
+            for k, v in kv.items():
+                fh.write(f"{k} = {v}\n")
+            fh.write("\n")

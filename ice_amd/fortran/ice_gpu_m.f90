@@ -1,0 +1,144 @@
+!> ISO_C_BINDING interface of the MI355X explicit residual-and-update path
+!> (include/ice_gpu.h).  A Fortran host that keeps ICE's block and boundary types
+!> calls these in place of the body of Explicit_Step
+!> (src/lib/numerics/time/explicit/Mod_Explicit.f90:11-245), passing its arrays
+!> with c_loc and their Fortran layouts unchanged.
+module ice_gpu_m
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: ICE_MK = 0, ICE_IG = 1, ICE_AG = 2
+  integer(c_int), parameter, public :: ICE_SAUREL = 0, ICE_RUSANOV = 1, ICE_HLLE = 2
+  integer(c_int), parameter, public :: ICE_MODE_PLAIN = 0, ICE_MODE_BATCHED = 1
+  integer(c_int), parameter, public :: ICE_REC_NI = 16, ICE_REC_ND = 7
+  !> record fields, 0-based as in the header (ICE_R_*, ICE_RD_*)
+  integer(c_int), parameter, public :: R_B = 0, R_I = 1, R_J = 2, R_K = 3, R_F = 4, R_TYPE = 5, R_P = 6, &
+    R_BS = 7, R_IS = 8, R_JS = 9, R_KS = 10, R_FS = 11, R_NORMAL_DIR = 12, R_NCHI1 = 13, R_NCHI2 = 14, R_DONOR0 = 15
+  integer(c_int), parameter, public :: RD_MASSFLUX = 0, RD_VELOCITY = 1, RD_TEMPERATURE = 2, RD_RADIUS = 3, &
+    RD_ALPHA = 4, RD_BETA = 5, RD_GFIXED = 6
+
+  !> ice_family_t
+  type, bind(C), public :: ice_family_t
+    integer(c_int32_t) :: closure = ICE_IG, riemann = ICE_RUSANOV, nfrozen = 0, frozen(4) = 0
+    integer(c_int32_t) :: evap_select = 0, intf_select = 0, surrogate = 0
+    real(c_double)     :: rho_mat = 0, cs = 0, lv = 0, emiss = 0, ep(8) = 0
+  end type
+
+  !> ice_config_t
+  type, bind(C), public :: ice_config_t
+    integer(c_int32_t) :: ngroups = 1, nrk = 2, time_accurate = 1, cfl_ramp_iter = 0, ndir = 3, limiter = 0
+    integer(c_int32_t) :: shock_detector = 0, irs = 0, coupled = 0, drag_select = 13, heat_select = 8
+    integer(c_int32_t) :: blow_select = 0, mode = ICE_MODE_PLAIN, var_yinf = 0, reserved(2) = 0
+    real(c_double)     :: cfl = 0.5d0, dt_max = 1.d-4, tau_factor = 1.d0, irs_beta = 0.5d0
+  end type
+
+  !> ice_status_t
+  type, bind(C), public :: ice_status_t
+    integer(c_int64_t) :: n_floored = 0
+    integer(c_int32_t) :: bad_state = 0, bad_block = 0, bad_i = 0, bad_j = 0, bad_k = 0, bad_p = 0
+    integer(c_int32_t) :: bad_recon = 0, iter = 0
+    real(c_double)     :: dtglobal = 0, time = 0
+  end type
+
+  public :: ice_gpu_create, ice_gpu_set_geometry, ice_gpu_set_boundaries, ice_gpu_put_state, ice_gpu_get_state, &
+            ice_gpu_put_gas, ice_gpu_put_tau, ice_gpu_step, ice_gpu_residual_norm, ice_gpu_timing, &
+            ice_gpu_last_error, ice_gpu_destroy, ice_gpu_error_text
+
+  interface
+    integer(c_int) function ice_gpu_create(device, cfg, fams, nb, dims, ctx) bind(C, name='ice_gpu_create')
+      import :: c_int, c_int32_t, c_ptr, ice_config_t, ice_family_t
+      integer(c_int), value      :: device, nb
+      type(ice_config_t)         :: cfg
+      type(ice_family_t)         :: fams(*)
+      integer(c_int32_t)         :: dims(*)
+      type(c_ptr)                :: ctx
+    end function
+    integer(c_int) function ice_gpu_set_geometry(ctx, b, vol, dl, m, face_i, face_j, face_k) &
+        bind(C, name='ice_gpu_set_geometry')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: b
+      real(c_double) :: vol(*), dl(*), m(*), face_i(*), face_j(*), face_k(*)
+    end function
+    integer(c_int) function ice_gpu_set_boundaries(ctx, nrec, reci, recd, ndonor, donor, weight) &
+        bind(C, name='ice_gpu_set_boundaries')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nrec, ndonor
+      integer(c_int32_t) :: reci(*), donor(*)
+      real(c_double) :: recd(*), weight(*)
+    end function
+    integer(c_int) function ice_gpu_put_state(ctx, b, p, prim) bind(C, name='ice_gpu_put_state')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: b, p
+      real(c_double) :: prim(*)
+    end function
+    integer(c_int) function ice_gpu_get_state(ctx, b, p, prim) bind(C, name='ice_gpu_get_state')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: b, p
+      real(c_double) :: prim(*)
+    end function
+    integer(c_int) function ice_gpu_put_gas(ctx, b, gprim, r, gam, k, mu) bind(C, name='ice_gpu_put_gas')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: b
+      real(c_double) :: gprim(*), r(*), gam(*), k(*), mu(*)
+    end function
+    integer(c_int) function ice_gpu_put_tau(ctx, b, p, tau) bind(C, name='ice_gpu_put_tau')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: b, p
+      real(c_double) :: tau(*)
+    end function
+    integer(c_int) function ice_gpu_step(ctx, nsteps, st) bind(C, name='ice_gpu_step')
+      import :: c_int, c_ptr, ice_status_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      type(ice_status_t) :: st
+    end function
+    integer(c_int) function ice_gpu_residual_norm(ctx, total, fam) bind(C, name='ice_gpu_residual_norm')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: total(5), fam(*)
+    end function
+    integer(c_int) function ice_gpu_timing(ctx, step_ms, kernel_ms, launches) bind(C, name='ice_gpu_timing')
+      import :: c_int, c_double, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: step_ms, kernel_ms
+      integer(c_int64_t) :: launches
+    end function
+    integer(c_int) function ice_gpu_last_error(ctx, buf, len) bind(C, name='ice_gpu_last_error')
+      import :: c_int, c_char, c_ptr
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: buf(*)
+      integer(c_int), value :: len
+    end function
+    subroutine ice_gpu_destroy(ctx) bind(C, name='ice_gpu_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+  end interface
+
+contains
+
+  !> The context's last message as a Fortran string
+  function ice_gpu_error_text(ctx) result(s)
+    type(c_ptr), intent(in) :: ctx
+    character(len=:), allocatable :: s
+    character(kind=c_char) :: buf(1024)
+    integer :: n, rc
+    buf = c_null_char
+    rc = ice_gpu_last_error(ctx, buf, 1024)
+    n = 0
+    do while (n < 1024)
+      if (buf(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    allocate(character(len=n) :: s)
+    s = transfer(buf(1:n), s)
+  end function
+
+end module ice_gpu_m

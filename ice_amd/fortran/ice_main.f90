@@ -1,0 +1,113 @@
+!> ICE's driver loop over the device path: set-up from the case directory's
+!> input.ini and INPUT/ files (Wrap_Setup.f90:48-121 for the parts the hot path
+!> reads), then Explicit_Step through the C-ABI until iter-threshold
+!> (main.f90:50-55, Wrap_Solve.f90:10-42), printing the residual norms every
+!> shell-diter iterations, and the solution in OUTPUT/part-field.tec.
+!>
+!> usage: ice_driver <case directory> [device]
+program ice_driver
+  use, intrinsic :: iso_c_binding
+  use, intrinsic :: iso_fortran_env, only: R8 => real64, I8 => int64, output_unit
+  use ice_gpu_m
+  use ice_host_m
+  implicit none
+  character(len=1024) :: dir, arg
+  type(registry_t) :: reg
+  type(host_block_t), allocatable :: blocks(:)
+  type(ice_config_t) :: cfg
+  type(ice_family_t), allocatable :: fams(:)
+  type(ice_status_t) :: st
+  integer(c_int32_t), allocatable :: reci(:,:), donor(:,:), dims(:)
+  real(R8), allocatable :: recd(:,:), weight(:), fam(:)
+  real(R8) :: rho_mat, cs, total(5), step_ms, kernel_ms
+  integer(c_int64_t) :: launches
+  type(c_ptr) :: ctx
+  integer :: device, nb, b, nc, ndir, iters, it, n, shell, rc, ndonor
+  integer(I8) :: cells, t0, t1, rate
+  character(len=96) :: names
+
+  call get_command_argument(1, dir)
+  if (len_trim(dir) == 0) call refuse('ice_driver', 'usage: ice_driver <case directory> [device]')
+  device = 0
+  call get_command_argument(2, arg)
+  if (len_trim(arg) > 0) read(arg, *) device
+
+  call read_registry(trim(dir)//'/input.ini', reg)
+  if (reg%nfam /= 1) call refuse('ice_driver', 'the initial condition file holds one family')
+  select case (trim(reg%closure(1)))
+  case ('MK'); nc = 6;  names = 'rho_p1 u_p1 v_p1 w_p1 T_p1 n_p1'
+  case ('IG'); nc = 7;  names = 'rho_p1 u_p1 v_p1 w_p1 P_p1 T_p1 n_p1'
+  case default; nc = 12; names = 'rho_p1 u_p1 v_p1 w_p1 P11_p1 P12_p1 P13_p1 P22_p1 P23_p1 P33_p1 T_p1 n_p1'
+  end select
+  call read_ic(trim(dir)//'/INPUT/part-ic.tec', nc, blocks)
+  call read_material(trim(dir)//'/INPUT/part-properties.dat', rho_mat, cs)
+  nb = size(blocks)
+  ndir = 0
+  do n = 1, 3
+    if (any([(blocks(b)%dim(n) > 1, b = 1, nb)])) ndir = ndir + 1
+  end do
+  call config_from_registry(reg, ndir, rho_mat, cs, cfg, fams)
+  call read_bc(trim(dir)//'/INPUT/part-bc.txt', blocks, cfg%ngroups, reci, recd, donor, weight)
+
+  allocate(dims(3 * nb))
+  cells = 0
+  do b = 1, nb
+    dims(3 * b - 2:3 * b) = blocks(b)%dim
+    cells = cells + product(int(blocks(b)%dim, I8))
+    call block_metrics(blocks(b))
+  end do
+
+  call check(ice_gpu_create(device, cfg, fams, nb, dims, ctx), 'create')
+  do b = 1, nb
+    call check(ice_gpu_set_geometry(ctx, b, blocks(b)%vol, blocks(b)%dl, blocks(b)%m, blocks(b)%fi, &
+                                    blocks(b)%fj, blocks(b)%fk), 'set_geometry')
+  end do
+  ndonor = 0
+  if (any(reci(R_TYPE, :) == 102)) ndonor = size(weight)
+  call check(ice_gpu_set_boundaries(ctx, size(reci, 2), reci, recd, ndonor, donor, weight), 'set_boundaries')
+  do b = 1, nb
+    call check(ice_gpu_put_state(ctx, b, 1, blocks(b)%prim), 'put_state')
+  end do
+
+  iters = reg_int(reg, 'iter-threshold')
+  shell = max(1, reg_int(reg, 'shell-diter'))
+  allocate(fam(5 * cfg%ngroups))
+  write(output_unit, '(A,I0,A,I0,A,I0,A)') ' ICE device path: ', nb, ' blocks, ', cells, ' cells, ', iters, ' iterations'
+  call system_clock(t0, rate)
+  it = 0
+  do while (it < iters)
+    n = min(shell, iters - it)
+    rc = ice_gpu_step(ctx, n, st)
+    if (st%bad_recon /= 0) call refuse('state_reconstruction', 'unphysical state at first order')
+    if (st%bad_state /= 0) then
+      write(output_unit, '(A,5(1X,I0))') ' [ERROR] [ICE::state_update] non-physical state at block, i, j, k, family', &
+        st%bad_block, st%bad_i, st%bad_j, st%bad_k, st%bad_p
+      error stop 1
+    end if
+    call check(rc, 'step')
+    it = it + n
+    call check(ice_gpu_residual_norm(ctx, total, fam), 'residual_norm')
+    write(output_unit, '(A,I8,A,5ES13.5)') ' iter', st%iter, '  res', total
+  end do
+  call system_clock(t1)
+  call check(ice_gpu_timing(ctx, step_ms, kernel_ms, launches), 'timing')
+  write(output_unit, '(A,F10.3,A,F12.2,A)') ' wall ', real(t1 - t0, R8) / rate, ' s, ', &
+    real(cells * cfg%ngroups * cfg%nrk, R8) * iters / (real(t1 - t0, R8) / rate) / 1e6_R8, ' Mcell-stage/s'
+  if (st%n_floored > 0) write(output_unit, '(A,I0)') ' [WARNING] floored densities: ', st%n_floored
+
+  do b = 1, nb
+    call check(ice_gpu_get_state(ctx, b, 1, blocks(b)%prim), 'get_state')
+  end do
+  call execute_command_line('mkdir -p '//trim(dir)//'/OUTPUT')
+  call write_solution(trim(dir)//'/OUTPUT/part-field.tec', names, blocks, st%iter)
+  call ice_gpu_destroy(ctx)
+
+contains
+
+  subroutine check(code, what)
+    integer(c_int), intent(in) :: code
+    character(len=*), intent(in) :: what
+    if (code /= 0) call refuse('ice_gpu_'//what, ice_gpu_error_text(ctx))
+  end subroutine
+
+end program ice_driver
