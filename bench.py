@@ -69,24 +69,84 @@ def cpu_baseline(case, sample_steps: int):
     return case.cell_stages(sample_steps) / dt / 1e6, dt
 
 
+def traffic_of(workload: str, mode: str, cells_per_launch: float):
+    """HBM bytes per launch of the stage kernel from the committed rocprofv3 PMC
+    summary of this workload (profiles/traffic_<workload>_<mode>.json: FETCH_SIZE
+    and WRITE_SIZE per launch, scaled by the factors a 1 GiB copy gives), scaled to
+    this run's cells per launch; None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{mode}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    return t["bytes_per_cell"] * cells_per_launch
+
+
+def surrogate_compare(gpu, args):
+    """The same workload on the plain path (the reference's Picard solve) and with
+    the converged solve without the network (secant from the equilibrium rate),
+    timed like the main run; the map evaluations of each."""
+    res = {}
+    for label, mode in (("plain", 0), ("no_network", 2)):
+        case = WORKLOADS[args.workload](native.MODE_PLAIN)
+        for f in case.fams:
+            f.surrogate = mode
+        r = cases.Runner(gpu, case, "ice_gpu_")
+        if args.warmup:
+            r.step(args.warmup)
+        t0 = time.perf_counter()
+        r.step(args.steps)
+        res[f"{label}_value"] = case.cell_stages(args.steps) / (time.perf_counter() - t0) / 1e6
+        r.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="jets2d", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="plain", choices=["plain", "batched"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batched", "surrogate"],
+                    help="surrogate: evaporation cases, the network-seeded interface solve (trained first, untimed)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB three times on the device (the PMC traffic calibration kernel) and exit")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--kernel-steps", type=int, default=2, help="steps of the per-kernel event pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copied 1 GiB three times (FETCH_SIZE / WRITE_SIZE reference)"}))
+        return
 
     mode = native.MODE_BATCHED if args.mode == "batched" else native.MODE_PLAIN
     case = WORKLOADS[args.workload](mode)
     gpu = native.load()
+    surrogate = None
+    if args.mode == "surrogate":
+        if not any(f.evap_select and f.intf_select == 1 for f in case.fams):
+            print(json.dumps({"error": "--mode surrogate needs a workload with a Langmuir-Knudsen interface solve"}))
+            sys.exit(2)
+        surrogate = surrogate_compare(gpu, args)
+        for f in case.fams:
+            f.surrogate = 1
     run = cases.Runner(gpu, case, "ice_gpu_")
+    if surrogate is not None:
+        rep = np.zeros(6, np.float32)
+        for p in range(1, case.cfg.ngroups + 1):
+            if gpu.ice_gpu_surrogate_train(run.ctx, p, 200, 512, 3e-3, 7, rep) != 0:
+                raise RuntimeError(run.error())
+        surrogate["training"] = {"samples": int(rep[0]), "loss_initial": float(rep[1]), "loss_final": float(rep[2]),
+                                 "rate_ratio_abs_err_mean": float(rep[3]), "rate_ratio_abs_err_max": float(rep[4]),
+                                 "train_ms": float(rep[5])}
     if args.warmup:
         run.step(args.warmup)
     t0 = time.perf_counter()
@@ -142,6 +202,13 @@ def main():
                      "launches": launches.value},
         "cpu_baseline": cpu,
     }
+    if surrogate is not None:
+        surrogate["value"] = value
+        surrogate["speedup_vs_plain"] = value / surrogate["plain_value"]
+        out["surrogate"] = surrogate
+    traffic = traffic_of(args.workload, args.mode, cells_per_launch)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic
     print(json.dumps(out))
 
 

@@ -175,12 +175,15 @@ ICE_DEV void lk_map(const GasSide& s, double XsEq, double LKd2, double md, doubl
   }
 }
 
-// lkCorrection :230-259: Picard on mdot, damped by one half when it does not contract
-ICE_DEV void lk_picard(const GasSide& s, double XsEq, double LKd2, double& md, double& Qd, bool& ovr) {
+// lkCorrection :230-259: Picard on mdot, damped by one half when it does not
+// contract; `evals` returns the map evaluations made
+ICE_DEV void lk_picard_counted(const GasSide& s, double XsEq, double LKd2, double& md, double& Qd, bool& ovr,
+                               int& evals) {
   double diffPrev = kHuge;
   double mdNew = 0.0, QdNew = 0.0;
   bool ovrNew = false;
   for (int it = 1; it <= 30; ++it) {
+    evals = it;
     lk_map(s, XsEq, LKd2, md, mdNew, QdNew, ovrNew);
     double diff = fabs(mdNew - md);
     if (diff <= 1.e-12 * fabs(mdNew)) {
@@ -222,9 +225,69 @@ ICE_DEV double lk_factor(const GasSide& s, const double* ep) {
   return 2.0 * s.mug * sqrt(2.0 * kPi * s.Tp * kRu / ep[kMv]) / (ep[kAlphaE] * s.Sc * s.p) / s.dp;
 }
 
-// evaporation :145-203 (Clausius-Clapeyron psat), the plain solve
+// The converged solve of lkCorrection's fixed point m = map(m), map the Picard
+// map of lk_map.  G(m) = map(m) - m is strictly decreasing on [md_eq, 0] (a more
+// negative rate lowers the surface vapour fraction and with it the rate), with
+// G(md_eq) >= 0 > G(0) = md_eq, so that interval brackets the single root.
+// Secant steps from the seed m0 and its Picard image, each kept inside the
+// bracket (else the bracket's midpoint), converged by lkCorrection's own test
+// (the map moves m by at most 1e-12 |map(m)|), whose map evaluation is then
+// the result.  False after kSeedEvals evaluations.
+constexpr int kSeedEvals = 16;
+ICE_DEV bool lk_secant(const GasSide& s, double XsEq, double LKd2, double md_eq, double m0, double& md, double& Qd,
+                       bool& ovr, int& evals) {
+  double lo = md_eq, hi = 0.0;
+  double f0, f1, Q;
+  bool o;
+  lk_map(s, XsEq, LKd2, m0, f0, Q, o);
+  evals = 1;
+  double g0 = f0 - m0;
+  if (fabs(g0) <= 1.e-12 * fabs(f0)) {
+    md = f0;
+    Qd = Q;
+    ovr = o;
+    return true;
+  }
+  if (g0 > 0.0) lo = m0;
+  else hi = m0;
+  double m1 = f0;
+  if (!(m1 > lo && m1 < hi)) m1 = 0.5 * (lo + hi);
+  while (evals < kSeedEvals) {
+    lk_map(s, XsEq, LKd2, m1, f1, Q, o);
+    ++evals;
+    double g1 = f1 - m1;
+    if (fabs(g1) <= 1.e-12 * fabs(f1)) {
+      md = f1;
+      Qd = Q;
+      ovr = o;
+      return true;
+    }
+    if (g1 > 0.0) lo = m1;
+    else hi = m1;
+    double den = g1 - g0;
+    double m2 = den != 0.0 ? m1 - g1 * (m1 - m0) / den : 0.5 * (lo + hi);
+    if (!(m2 > lo && m2 < hi)) m2 = 0.5 * (lo + hi);
+    m0 = m1;
+    g0 = g1;
+    m1 = m2;
+  }
+  return false;
+}
+
+// Counters of the interface solves of a stage (optional): solves, map
+// evaluations, fallbacks to the plain solve, largest evaluation count.
+struct LkStats {
+  unsigned long long* c;  // [4] or nullptr
+};
+
+// evaporation :145-203 (Clausius-Clapeyron psat).  seed <= 0: the plain solve,
+// the reference's damped Picard iteration from the equilibrium rate, at most 30
+// evaluations (it can stop short of the fixed point where the map's slope is
+// near -1).  seed > 0, the converged solve: lk_secant from seed x (equilibrium
+// rate), the plain solve when it does not converge.
 ICE_DEV void evaporation_d(double rhog, double Tg, double gamma, double Rg, double mug, double kg, double Tp, double dp,
-                           double Re, int evap, int intf, const double* ep, double& mdot, double& Qd, bool& ovr) {
+                           double Re, int evap, int intf, const double* ep, double& mdot, double& Qd, bool& ovr,
+                           float seed = 0.0f, LkStats stats = LkStats{nullptr}) {
   mdot = 0.0;
   Qd = 0.0;
   ovr = false;
@@ -235,7 +298,45 @@ ICE_DEV void evaporation_d(double rhog, double Tg, double gamma, double Rg, doub
   if (!evap_surface(s, rhog, Tg, gamma, Rg, mug, kg, Tp, dp, Re, ep, Xs, Ys, BM)) return;
   gas_side_setup(s, ep);
   gas_side_rate_d(s, Ys, BM, mdot, Qd, ovr);
-  if (intf == 1) lk_picard(s, Xs, lk_factor(s, ep), mdot, Qd, ovr);
+  if (intf != 1) return;
+  int evals = 0;
+  bool fell = false;
+  if (seed > 0.0f) {
+    double md = 0.0, Q = 0.0;
+    bool o = false;
+    if (lk_secant(s, Xs, lk_factor(s, ep), mdot, (double)seed * mdot, md, Q, o, evals)) {
+      mdot = md;
+      Qd = Q;
+      ovr = o;
+    } else {
+      fell = true;
+    }
+  }
+  if (seed <= 0.0f || fell) {
+    int it = 0;
+    lk_picard_counted(s, Xs, lk_factor(s, ep), mdot, Qd, ovr, it);
+    evals += it;
+  }
+  if (stats.c) {
+    atomicAdd(&stats.c[0], 1ull);
+    atomicAdd(&stats.c[1], (unsigned long long)evals);
+    if (fell) atomicAdd(&stats.c[2], 1ull);
+    atomicMax(&stats.c[3], (unsigned long long)evals);
+  }
+}
+
+// Whether evaporation() will run the interface solve for this drop (the
+// surface state admits evaporation and the interface is Langmuir-Knudsen),
+// and the gas pressure it uses.
+ICE_DEV bool lk_needed(double rhog, double Tg, double Rg, double Tp, int evap, int intf, const double* ep, double& p) {
+  p = rhog * Rg * Tg;
+  if (evap == 0 || intf != 1) return false;
+  double Mg = kRu / Rg;
+  double psat = kPatm * exp(-ep[kLvMvRu] * (1.0 / Tp - ep[kInvTb]));
+  double Xs = psat >= p ? 1.0 - kXsCap : dmin2(psat / p, 1.0 - kXsCap);
+  double Ys = molar2mass_d(Xs, ep[kMv], Mg);
+  if (Ys <= ep[kYinf]) return false;
+  return (Ys - ep[kYinf]) / (1.0 - Ys) > 0.0;
 }
 
 // blowingFactor :129-143

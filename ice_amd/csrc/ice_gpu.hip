@@ -33,6 +33,15 @@ struct FamDev {
   int nc = 0;
 };
 
+// The learned seed of a family: FP32 master weights, Adam moments, BF16
+// copies, gradient partials, interface-solve counters
+struct SurrDev {
+  float *P = nullptr, *m = nullptr, *v = nullptr, *part = nullptr, *loss = nullptr;
+  uint16_t* w = nullptr;
+  unsigned long long* stats = nullptr;
+  int nparts = 0, on = 0;
+};
+
 // host view of a record and its pairing (ICE_bc_type)
 struct HRec {
   int v[ICE_REC_NI];
@@ -68,6 +77,8 @@ struct ice_gpu_ctx {
   std::vector<hipEvent_t> kev;
   std::vector<FamArgs*> kev_args;  // FamArgs arrays of eager batched steps, freed with the events
   BatchedPlan* batched = nullptr;
+  std::vector<SurrDev> surr;   // per family
+  int lk_stats_on = 0;
   int tile[3] = {-1, -1, -1};  // cells per stage tile; -1 default shape, 0 per-cell kernel
   std::string err;
 };
@@ -294,6 +305,13 @@ FamArgs fam_args(ice_gpu_ctx* c, int p) {
   a.keep_source = c->keep_source; a.var_yinf = c->cfg.var_yinf; a.iter = c->iter;
   a.cfl_ramp_iter = c->cfg.cfl_ramp_iter; a.ndir = c->cfg.ndir; a.fam = p;
   a.cfl = c->cfg.cfl; a.dt_max = c->cfg.dt_max; a.tau_factor = c->cfg.tau_factor; a.irs_beta = c->cfg.irs_beta;
+  const SurrDev& S = c->surr[p];
+  a.mlp_on = S.on;
+  if (S.on) {
+    a.mlp.w1t = S.w; a.mlp.w2t = S.w + 1024; a.mlp.w3t = S.w + 5120; a.mlp.w2n = S.w + 6144; a.mlp.w3n = S.w + 10240;
+    a.mlp.b1 = S.P + kOffB1; a.mlp.b2 = S.P + kOffB2; a.mlp.b3 = S.P + kOffB3;
+  }
+  a.lk_stats = c->lk_stats_on ? S.stats : nullptr;
   return a;
 }
 
@@ -521,6 +539,7 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
     }
   }
   c->fam.resize(cfg->ngroups);
+  c->surr.resize(cfg->ngroups);
   for (int p = 0; p < cfg->ngroups; ++p) {
     FamDev& F = c->fam[p];
     F.nc = nc_of(fams[p].closure);
@@ -924,6 +943,9 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
     F(f.beta); F(f.dt); F(f.tau); F(f.src); F(f.R); F(f.T); F(f.bsum); F(f.connF); F(f.inj); F(f.bidx);
     F(f.grp); F(f.own); F(f.bfirst); F(f.brec); F(f.bcell_g);
   }
+  for (auto& S : c->surr) {
+    F(S.P); F(S.m); F(S.v); F(S.part); F(S.loss); F(S.w); F(S.stats);
+  }
   batched_destroy(c);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1179,3 +1201,6 @@ void batched_destroy(ice_gpu_ctx* c) {
   delete P;
   c->batched = nullptr;
 }
+
+// the learned seed of the interface solve (same translation unit: it uses the context)
+#include "surrogate.hpp"

@@ -7,6 +7,7 @@
 #pragma once
 #include "dev_physics.hpp"
 #include "dev_evap.hpp"
+#include "dev_mlp.hpp"
 
 namespace ice {
 
@@ -85,6 +86,10 @@ struct FamArgs {
   int nfrozen, frozen[4];
   int keep_source, last_stage, var_yinf, iter, cfl_ramp_iter, ndir, fam;
   double cfl, dt_max, tau_factor, irs_beta;
+  // learned seed of the interface solve (mat.surrogate = 1) and its counters
+  MlpDev mlp;
+  int mlp_on;
+  unsigned long long* lk_stats;  // [4] or nullptr
 };
 
 ICE_DEV long long gidx(const BlkDev& B, int i, int j, int k) {
@@ -132,8 +137,19 @@ ICE_DEV bool face_acc(const FamArgs& a, long long g, long long stride, int d, in
 // ITC surface (Lib_Drop_Surface.f90:15-43), evaporation with the interface and
 // blowing models (Lib_Evaporation.f90), no combustion, no P2T/boiling.  With a
 // per-cell far-field vapour fraction its value replaces ep(iYinf).
+// The drop radius, slip speed and slip Reynolds number of compute_source_
 template <int M>
-ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double* src, double& tau) {
+ICE_DEV void drop_state(const FamArgs& a, long long g, const double* q, const double* gp, double& Rp, double& slip,
+                        double& Re) {
+  constexpr int NC = Closure<M>::NC;
+  Rp = pow(0.75 * q[0] / (q[NC - 1] * kPi * a.mat.rho), 1.0 / 3.0);
+  slip = norm3(gp[1] - q[1], gp[2] - q[2], gp[3] - q[3]);
+  Re = 2.0 * gp[0] * Rp * slip / a.gMu[g];
+}
+
+// seed > 0: the interface solve starts from seed x (equilibrium rate), see evaporation_d
+template <int M>
+ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double* src, double& tau, float seed = 0.0f) {
   constexpr int NC = Closure<M>::NC;
   const long long N = a.N;
   double gp[5];
@@ -163,7 +179,8 @@ ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double*
 #pragma unroll
     for (int m = 0; m < 8; ++m) ep[m] = mt.ep[m];
     if (a.var_yinf) ep[kYinf] = a.yinf[g];
-    evaporation_d(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, mt.evap, mt.intf, ep, mdot, Qevap, ovr);
+    evaporation_d(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, mt.evap, mt.intf, ep, mdot, Qevap, ovr, seed,
+                  LkStats{a.lk_stats});
   }
   if (ovr) Qconv = Qevap;
   else if (a.blow_sel == 1)

@@ -7,6 +7,57 @@ namespace ice {
 
 constexpr int kThreads = 256;
 
+// ------------------------------------------------------------ learned seed
+// LDS of the seed per workgroup: per wave 64x16 inputs, 64x64 hidden (BF16)
+// and 64x2 outputs (FP32).
+constexpr int kMlpWaveLds = 64 * 16 + 64 * 64 + 64 * 2 * 2;  // in uint16_t
+constexpr int kMlpLds = kMlpWaveLds * (kThreads / 64);
+
+// Seed of the interface solve of the calling thread's cell.  mat.surrogate:
+// 0 the plain solve (0), 2 the equilibrium rate (1), 1 the network's rate ratio
+// for the cell, clamped to [1e-2, 1].  With 1, every thread of the workgroup
+// calls this (the wave's 64 lanes feed one MLP pass on the MFMA units, skipped
+// when no lane's cell needs the solve).
+template <int M>
+ICE_DEV float stage_seed(const FamArgs& a, bool has_cell, long long g, uint16_t* lds) {
+  constexpr int NC = Closure<M>::NC;
+  if (a.mat.surrogate == 2) return 1.0f;
+  if (a.mat.surrogate != 1 || !a.mlp_on || a.mat.evap == 0 || a.mat.intf != 1) return 0.0f;
+  bool need = false;
+  float f[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (has_cell) {
+    double q[NC], gp[5];
+    load_cell<NC>(a.cur, a.N, g, q);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) gp[m] = a.gas[(long long)m * a.N + g];
+    double ep[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) ep[m] = a.mat.ep[m];
+    if (a.var_yinf) ep[kYinf] = a.yinf[g];
+    double p;
+    need = lk_needed(gp[0], gp[4], a.gR[g], q[NC - 2], a.mat.evap, a.mat.intf, ep, p);
+    if (need) {
+      double Rp, slip, Re;
+      drop_state<M>(a, g, q, gp, Rp, slip, Re);
+      mlp_features(gp[4], p, ep[kYinf], q[NC - 2], 2.0 * Rp, Re, f);
+    }
+  }
+  if (!__any(need)) return 0.0f;
+  const int l = threadIdx.x & 63;
+  uint16_t* x = lds + (threadIdx.x >> 6) * kMlpWaveLds;
+  uint16_t* h = x + 64 * 16;
+  float* out = (float*)(h + 64 * 64);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[l * 16 + k] = k < kMlpIn ? f2bf(f[k]) : (uint16_t)0;
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  mlp_forward_wave(a.mlp, x, h, h, out);
+  float r = out[l * kMlpOut];
+  r = r < 1e-2f ? 1e-2f : (r > 1.0f ? 1.0f : r);
+  return need ? r : 0.0f;
+}
+
+
 ICE_DEV void cell_of(const BlkDev& B, long long t, int& i, int& j, int& k) {
   i = (int)(t % B.ni) + 1;
   long long r = t / B.ni;
@@ -480,10 +531,15 @@ __global__ void __launch_bounds__(kThreads) k_stage(FamArgs a, int b) {
   const BlkDev& B = a.blk[b];
   const BlkDev Bv = B;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= Bv.cells) return;
-  int i, j, k;
-  cell_of(Bv, t, i, j, k);
+  int i = 1, j = 1, k = 1;
+  if (t < Bv.cells) cell_of(Bv, t, i, j, k);
   long long g = gidx(Bv, i, j, k);
+  float seed = 0.0f;
+  if constexpr (COUPLED) {
+    __shared__ __attribute__((aligned(16))) uint16_t mlds[kMlpLds];
+    seed = stage_seed<M>(a, t < Bv.cells, g, mlds);
+  }
+  if (t >= Bv.cells) return;
   double r[NC];
 #pragma unroll
   for (int v = 0; v < NC; ++v) r[v] = 0.0;
@@ -516,7 +572,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(FamArgs a, int b) {
   if constexpr (COUPLED) {
     double q[NC], tau;
     load_cell<NC>(a.cur, a.N, g, q);
-    cell_source<M>(a, g, q, src, tau);
+    cell_source<M>(a, g, q, src, tau, seed);
     for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
     if (a.last_stage) {
       a.tau[g] = tau;
@@ -601,7 +657,7 @@ constexpr int kMaxTileFaces = kThreads;
 // kernel for the tiles it takes from its work queue.  fl is the workgroup's
 // LDS buffer of face fluxes.
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
-ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl) {
+ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl, uint16_t* mlds) {
   constexpr int NC = Closure<M>::NC;
   const BlkDev& Bref = a.blk[b];
   const BlkDev B = Bref;
@@ -674,12 +730,14 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
     __syncthreads();
   }
   if (!ok) atomicOr(&a.st->bad_recon, 1);
+  float seed = 0.0f;
+  if constexpr (COUPLED) seed = stage_seed<M>(a, has_cell, g, mlds);
   if (!has_cell) return;
   double src[NC];
   if constexpr (COUPLED) {
     double q[NC], tau;
     load_cell<NC>(a.cur, a.N, g, q);
-    cell_source<M>(a, g, q, src, tau);
+    cell_source<M>(a, g, q, src, tau, seed);
     for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
     if (a.last_stage) {
       a.tau[g] = tau;
@@ -701,11 +759,24 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
   }
 }
 
+// Workgroup w of a launch runs on XCD w % 8 (workgroups are dealt round-robin
+// over the 8 XCDs): give each XCD one contiguous run of the n tiles, so that
+// tiles sharing halo cells share that XCD's L2.
+ICE_DEV int xcd_tile(int w, int n) {
+#ifdef ICE_NO_XCD_MAP
+  return w;
+#endif
+  constexpr int kXcd = 8;
+  int q = n / kXcd, r = n % kXcd, x = w % kXcd, j = w / kXcd;
+  return x * q + (x < r ? x : r) + j;
+}
+
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads) k_stage_tiled(FamArgs a, int b, Tile T) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
-  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(a, b, T, blockIdx.x, fl);
+  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
+  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(a, b, T, xcd_tile(blockIdx.x, gridDim.x), fl, mlds);
 }
 
 // ------------------------------------------------------------------ batched path
@@ -743,13 +814,14 @@ template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads) k_stage_persist(const FamArgs* fams, WorkQueues Q) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
+  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
   __shared__ int slot;
   int q = blockIdx.x % Q.nq;
   for (;;) {
     int it = next_item(Q, q, &slot);
     if (it < 0) break;
     const WorkItem w = Q.items[it];
-    stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl);
+    stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
   }
 }
 

@@ -57,6 +57,9 @@ class Status(C.Structure):
 
 _DP = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _IP = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_FP = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_LP = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+MLP_PARAMS = 6288
 
 # symbols of include/ice_gpu.h with their argument types
 SIGNATURES = {
@@ -76,6 +79,13 @@ SIGNATURES = {
     "ice_gpu_timing": [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "ice_gpu_set_kernel_timing": [C.c_void_p, C.c_int],
     "ice_gpu_set_tiles": [C.c_void_p, C.c_int, C.c_int, C.c_int],
+    "ice_gpu_mlp_grad": [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP, _FP],
+    "ice_gpu_mlp_forward": [C.c_void_p, C.c_int, _FP, _FP, _FP],
+    "ice_gpu_surrogate_train": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _FP],
+    "ice_gpu_surrogate_params": [C.c_void_p, C.c_int, _FP],
+    "ice_gpu_surrogate_set_params": [C.c_void_p, C.c_int, _FP],
+    "ice_gpu_set_lk_stats": [C.c_void_p, C.c_int],
+    "ice_gpu_lk_stats": [C.c_void_p, C.c_int, _LP],
     "ice_gpu_last_error": [C.c_void_p, C.c_char_p, C.c_int],
     "ice_gpu_destroy": [C.c_void_p],
 }

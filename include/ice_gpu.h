@@ -163,6 +163,37 @@ int  ice_gpu_set_tiles(ice_gpu_ctx* ctx, int tx, int ty, int tz);
 /* 1: bracket every launch of the dominant kernel with HIP events (bench only). */
 int  ice_gpu_set_kernel_timing(ice_gpu_ctx* ctx, int on);
 
+/* ---- learned seed of the Langmuir-Knudsen interface solve (no counterpart in the
+ * reference: the north star's surrogate).  A 6-64-64-2 ReLU network in BF16 on the
+ * MFMA units, FP32 accumulation; parameters are ICE_MLP_PARAMS floats:
+ * W1t[64][16] b1[64] W2t[64][64] b2[64] W3t[16][64] b3[16] (W[n][k], input k to
+ * output n; inputs padded 6 -> 16 and outputs 2 -> 16 with zeros).  Inputs are the
+ * six scaled features of a drop (gas T, pressure, far-field vapour fraction, drop T,
+ * diameter, slip Reynolds number); outputs the interface state: the rate over the
+ * equilibrium rate and the surface vapour molar fraction over its equilibrium value.
+ * ice_family_t.surrogate = 1 seeds each cell's interface solve with the first output
+ * (secant iteration, the reference's Picard solve as fallback); 2 starts the same
+ * secant iteration from the equilibrium rate (the method without the network). */
+#define ICE_MLP_PARAMS 6288
+/* Loss (mean squared error) and its gradient for n samples x[n][6] -> y[n][2]. */
+int  ice_gpu_mlp_grad(ice_gpu_ctx* ctx, int n, const float* x, const float* y, const float* params, float* grad,
+                      float* loss);
+/* Network outputs out[n][2] for x[n][6]. */
+int  ice_gpu_mlp_forward(ice_gpu_ctx* ctx, int n, const float* x, const float* params, float* out);
+/* Samples family p's cells that run the interface solve in the current state, solves
+ * them with the plain solver, and trains the family's network on them on the GPU
+ * (Adam, mini-batches of `batch`, `epochs` passes, learning rate lr, initial weights
+ * from `seed`).  report[6]: samples, initial loss, final loss, mean and max absolute
+ * error of the rate ratio, milliseconds. */
+int  ice_gpu_surrogate_train(ice_gpu_ctx* ctx, int p, int epochs, int batch, double lr, uint64_t seed,
+                             float* report);
+int  ice_gpu_surrogate_params(ice_gpu_ctx* ctx, int p, float* params);
+int  ice_gpu_surrogate_set_params(ice_gpu_ctx* ctx, int p, const float* params);
+/* 1: count the interface solves of every stage; ice_gpu_lk_stats: family p's solves,
+ * map evaluations, fallbacks to the plain solve and the largest evaluation count. */
+int  ice_gpu_set_lk_stats(ice_gpu_ctx* ctx, int on);
+int  ice_gpu_lk_stats(ice_gpu_ctx* ctx, int p, int64_t* counts);
+
 int  ice_gpu_last_error(ice_gpu_ctx* ctx, char* buf, int len);
 void ice_gpu_destroy(ice_gpu_ctx* ctx);
 

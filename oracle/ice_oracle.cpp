@@ -18,6 +18,7 @@
 #include "../include/ice_gpu.h"
 #include "oracle_physics.hpp"
 #include "oracle_evap.hpp"
+#include "oracle_mlp.hpp"
 
 using namespace orc;
 
@@ -28,6 +29,8 @@ static const int GUIDE[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0
 struct Family {
   int model, solver, nc, nbase, nfrozen, frozen[4];
   Mat mat;
+  int surrogate = 0;          // 1: network seed, 2: equilibrium seed
+  std::vector<float> mlp;     // the network's parameters (surrogate = 1)
 };
 
 struct Rec {
@@ -594,7 +597,20 @@ void source_cell(orc_ctx* c, const Family& F, const double* q, const double* gp,
   double ep[8];
   for (int m = 0; m < 8; ++m) ep[m] = F.mat.ep[m];
   if (c->cfg.var_yinf) ep[EP_YINF] = yinf;
-  evaporation(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, F.mat.evap, F.mat.intf, ep, mdot, Qevap, ovr);
+  double seed = 0.0;
+  if (F.surrogate == 2) {
+    seed = 1.0;
+  } else if (F.surrogate == 1 && !F.mlp.empty() && F.mat.evap != 0 && F.mat.intf == 1) {
+    // the network's rate ratio, clamped as the device clamps it
+    float x[MLP_IN];
+    mlp_features(gp[4], gp[0] * R * gp[4], ep[EP_YINF], Ts, 2.0 * Rp, Re, x);
+    MlpCache mc;
+    mlp_forward(F.mlp.data(), x, mc);
+    float r = mc.out[0];
+    seed = r < 1e-2f ? 1e-2f : (r > 1.0f ? 1.0f : r);
+  }
+  evaporation(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, F.mat.evap, F.mat.intf, ep, mdot, Qevap, ovr, nullptr,
+              seed);
   if (ovr) Qconv = Qevap;
   else if (c->cfg.blow_select == 1)
     Qconv = Qconv * blowing_factor(gam, R, mu, gk, rho_mat, 2.0 * Rp, 4.0 / 3.0 * PI * (Rp * Rp * Rp) * rho_mat, mdot);
@@ -884,6 +900,7 @@ int orc_create(int /*device*/, const ice_config_t* cfg, const ice_family_t* fams
     F.mat.rho = fams[p].rho_mat; F.mat.cs = fams[p].cs; F.mat.lv = fams[p].lv; F.mat.emiss = fams[p].emiss;
     for (int m = 0; m < 8; ++m) F.mat.ep[m] = fams[p].ep[m];
     F.mat.evap = fams[p].evap_select; F.mat.intf = fams[p].intf_select;
+    F.surrogate = fams[p].surrogate;
     c->fam.push_back(F);
   }
   const double nan = std::numeric_limits<double>::quiet_NaN();
@@ -1071,6 +1088,27 @@ int orc_last_error(orc_ctx* c, char* buf, int len) {
   return 0;
 }
 
+// the network of the seeded interface solve (surrogate = 1)
+int orc_surrogate_set_params(orc_ctx* c, int p, const float* params) {
+  if (p < 1 || p > (int)c->fam.size()) return 1;
+  c->fam[p - 1].mlp.assign(params, params + MLP_PARAMS);
+  return 0;
+}
+int orc_mlp_grad(orc_ctx* c, int n, const float* x, const float* y, const float* params, float* grad, float* loss) {
+  (void)c;
+  if (n < 1) return 1;
+  *loss = mlp_grad(params, n, x, y, grad);
+  return 0;
+}
+int orc_mlp_forward(orc_ctx* c, int n, const float* x, const float* params, float* out) {
+  (void)c;
+  MlpCache mc;
+  for (int s = 0; s < n; ++s) {
+    mlp_forward(params, x + (size_t)s * MLP_IN, mc);
+    for (int o = 0; o < MLP_OUT; ++o) out[(size_t)s * MLP_OUT + o] = mc.out[o];
+  }
+  return 0;
+}
 void orc_destroy(orc_ctx* c) { delete c; }
 
 }  // extern "C"

@@ -198,11 +198,68 @@ static inline void lk_correction(const EvapState& s, double XsEq, double& md, do
   ovr = ovrNew;
 }
 
+// One Picard map of lkCorrection (:236-244): the gas-side rate at the surface
+// state that the rate md implies
+static inline void lk_map(const EvapState& s, double XsEq, double LKd2, double md, double& mdNew, double& QdNew,
+                          bool& ovrNew) {
+  const double* ep = s.ep;
+  double beta = -md * s.Pr / (2.0 * PI * s.mug * s.dp);
+  double XsN = fmax2(XsEq - LKd2 * beta, 0.0);
+  double YsN = molar2mass(XsN, ep[EP_MV], s.Mg);
+  mdNew = 0.0;
+  QdNew = 0.0;
+  ovrNew = false;
+  if (YsN > ep[EP_YINF]) {
+    double BMN = (YsN - ep[EP_YINF]) / (1.0 - YsN);
+    if (BMN > 0.0) gas_side_rate(s, YsN, BMN, mdNew, QdNew, ovrNew);
+  }
+}
+
+// The converged solve of the surrogate path (no counterpart in the reference):
+// G(m) = map(m) - m decreases strictly on [md_eq, 0], G(md_eq) >= 0 > G(0), so
+// that interval brackets its root; secant steps from the seed m0 and its
+// Picard image, kept inside the bracket (else its midpoint), converged by
+// lkCorrection's own test; false after SEED_EVALS evaluations.
+static const int SEED_EVALS = 16;
+static inline bool lk_secant(const EvapState& s, double XsEq, double LKd2, double md_eq, double m0, double& md,
+                             double& Qd, bool& ovr, int& evals) {
+  double lo = md_eq, hi = 0.0;
+  double f0, f1, Q;
+  bool o;
+  lk_map(s, XsEq, LKd2, m0, f0, Q, o);
+  evals = 1;
+  double g0 = f0 - m0;
+  if (std::fabs(g0) <= 1.e-12 * std::fabs(f0)) {
+    md = f0; Qd = Q; ovr = o;
+    return true;
+  }
+  if (g0 > 0.0) lo = m0;
+  else hi = m0;
+  double m1 = f0;
+  if (!(m1 > lo && m1 < hi)) m1 = 0.5 * (lo + hi);
+  while (evals < SEED_EVALS) {
+    lk_map(s, XsEq, LKd2, m1, f1, Q, o);
+    ++evals;
+    double g1 = f1 - m1;
+    if (std::fabs(g1) <= 1.e-12 * std::fabs(f1)) {
+      md = f1; Qd = Q; ovr = o;
+      return true;
+    }
+    if (g1 > 0.0) lo = m1;
+    else hi = m1;
+    double den = g1 - g0;
+    double m2 = den != 0.0 ? m1 - g1 * (m1 - m0) / den : 0.5 * (lo + hi);
+    if (!(m2 > lo && m2 < hi)) m2 = 0.5 * (lo + hi);
+    m0 = m1; g0 = g1; m1 = m2;
+  }
+  return false;
+}
+
 // evaporation :145-203 (no property-table psat): mdot [kg/s] <= 0, Qd [W], ovr
 // when the model's gas-side heat replaces the Nusselt one
 static inline void evaporation(double rhog, double Tg, double gamma, double Rg, double mug, double kg, double Tp,
                                double dp, double Re, int evap, int intf, const double* ep, double& mdot, double& Qd,
-                               bool& ovr, int* iters = nullptr) {
+                               bool& ovr, int* iters = nullptr, double seed = 0.0) {
   mdot = 0.0;
   Qd = 0.0;
   ovr = false;
@@ -224,7 +281,22 @@ static inline void evaporation(double rhog, double Tg, double gamma, double Rg, 
   s.Pr = mug * s.cpg / kg;
   s.Sc = s.Pr * ep[EP_LE];
   gas_side_rate(s, Ys, BM, mdot, Qd, ovr);
-  if (intf == 1) lk_correction(s, Xs, mdot, Qd, ovr, iters);
+  if (intf != 1) return;
+  if (seed > 0.0) {
+    // the surrogate path: seeded secant, the plain solve when it does not converge
+    double LKd2 = 2.0 * s.mug * std::sqrt(2.0 * PI * s.Tp * RU / ep[EP_MV]) / (ep[EP_ALPHAE] * s.Sc * s.p) / s.dp;
+    double md, Q;
+    bool o;
+    int ev = 0;
+    bool ok = lk_secant(s, Xs, LKd2, mdot, seed * mdot, md, Q, o, ev);
+    int ev2 = 0;
+    if (ok) {
+      mdot = md; Qd = Q; ovr = o;
+      if (iters) *iters = ev + ev2;
+      return;
+    }
+  }
+  lk_correction(s, Xs, mdot, Qd, ovr, iters);
 }
 
 // blowingFactor :129-143
