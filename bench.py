@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--workload", default="jets2d", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="plain", choices=["plain", "batched", "surrogate"],
                     help="surrogate: evaporation cases, the network-seeded interface solve (trained first, untimed)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default stage tile (otherwise the library's autotuner picks it, untimed)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB three times on the device (the PMC traffic calibration kernel) and exit")
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0: skip)")
@@ -147,6 +149,14 @@ def main():
         surrogate["training"] = {"samples": int(rep[0]), "loss_initial": float(rep[1]), "loss_final": float(rep[2]),
                                  "rate_ratio_abs_err_mean": float(rep[3]), "rate_ratio_abs_err_max": float(rep[4]),
                                  "train_ms": float(rep[5])}
+    tuned = None
+    if not args.no_autotune:
+        best = np.zeros(4, np.int32)
+        trial_ms = np.zeros(16)
+        if gpu.ice_gpu_autotune(run.ctx, 2, best, trial_ms) != 0:
+            raise RuntimeError(run.error())
+        tuned = {"tile": best[:3].tolist(), "dispatch": "items" if best[3] else "queues",
+                 "trial_ms_per_step": [round(float(x), 4) for x in trial_ms if np.isfinite(x) and x > 0]}
     if args.warmup:
         run.step(args.warmup)
     t0 = time.perf_counter()
@@ -194,7 +204,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": DESCR[args.workload], "mode": args.mode, "cells": case.cells,
                    "families": case.cfg.ngroups, "rk_stages": case.cfg.nrk,
-                   "event_ms_per_step": step_event_ms / args.steps, "iter": st.iter},
+                   "event_ms_per_step": step_event_ms / args.steps, "iter": st.iter, "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                      "kernel": "k_stage_tiled / k_stage_persist (fused interior flux + source + residual + RK update)",

@@ -80,6 +80,7 @@ struct ice_gpu_ctx {
   std::vector<SurrDev> surr;   // per family
   int lk_stats_on = 0;
   int tile[3] = {-1, -1, -1};  // cells per stage tile; -1 default shape, 0 per-cell kernel
+  int dispatch = 0;            // batched stage: 0 persistent queues, 1 one workgroup per item
   std::string err;
 };
 
@@ -899,10 +900,12 @@ int ice_gpu_timing(ice_gpu_ctx* c, double* step_ms, double* kernel_ms, int64_t* 
 int ice_gpu_set_tiles(ice_gpu_ctx* c, int tx, int ty, int tz) {
   if (tx == 0) {
     c->tile[0] = c->tile[1] = c->tile[2] = 0;
+    batched_destroy(c);
     return 0;
   }
   if (tx < 0) {
     c->tile[0] = c->tile[1] = c->tile[2] = -1;
+    batched_destroy(c);
     return 0;
   }
   for (const BlkDev& B : c->blk) {
@@ -915,6 +918,131 @@ int ice_gpu_set_tiles(ice_gpu_ctx* c, int tx, int ty, int tz) {
                   tx, ty, tz, kThreads);
   }
   c->tile[0] = tx; c->tile[1] = ty; c->tile[2] = tz;
+  batched_destroy(c);  // the batched plan's work items are tiles
+  return 0;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Candidate stage tiles (cells along i, j, k) for blocks resolving 3, 2 and 1
+// directions; each must keep its cells and the faces of each direction within
+// the 256 threads of a workgroup (checked by ice_gpu_set_tiles).
+namespace {
+const int kTiles3[][3] = {{7, 6, 5}, {6, 6, 6}, {7, 7, 4}, {10, 5, 4}, {12, 4, 4}, {15, 4, 3}, {14, 6, 2}, {8, 4, 7}};
+const int kTiles2[][3] = {{15, 16, 1}, {16, 15, 1}, {25, 9, 1}, {12, 20, 1}, {20, 12, 1}, {10, 24, 1}};
+const int kTiles1[][3] = {{255, 1, 1}, {128, 1, 1}, {192, 1, 1}};
+
+// a device copy of everything a step changes, to run trial steps and put it back
+struct Snapshot {
+  std::vector<double*> buf;  // per family: 3 state buffers, tau, dt
+  std::vector<int> cur, old;
+  DevStatus st;
+  int iter = 0;
+};
+
+int snapshot(ice_gpu_ctx* c, Snapshot& S, bool take) {
+  const size_t N = (size_t)c->N;
+  if (take) {
+    S.buf.clear();
+    for (auto& F : c->fam) {
+      for (int m = 0; m < 5; ++m) {
+        double* src = m < 3 ? F.buf[m] : (m == 3 ? F.tau : F.dt);
+        size_t n = (m < 3 ? (size_t)F.nc : 1) * N;
+        double* d = nullptr;
+        if (src) {
+          if (dalloc(c, &d, n)) return 1;
+          HIPCHK(hipMemcpyAsync(d, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        }
+        S.buf.push_back(d);
+      }
+      S.cur.push_back(F.cur);
+      S.old.push_back(F.old);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&S.st, c->d_st, sizeof S.st, hipMemcpyDeviceToHost));
+    S.iter = c->iter;
+    return 0;
+  }
+  for (size_t p = 0; p < c->fam.size(); ++p) {
+    FamDev& F = c->fam[p];
+    for (int m = 0; m < 5; ++m) {
+      double* dst = m < 3 ? F.buf[m] : (m == 3 ? F.tau : F.dt);
+      size_t n = (m < 3 ? (size_t)F.nc : 1) * N;
+      if (dst) HIPCHK(hipMemcpyAsync(dst, S.buf[p * 5 + m], n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    F.cur = S.cur[p];
+    F.old = S.old[p];
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->d_st, &S.st, sizeof S.st, hipMemcpyHostToDevice));
+  c->iter = S.iter;
+  return 0;
+}
+
+void snapshot_free(Snapshot& S) {
+  for (double* d : S.buf)
+    if (d) (void)hipFree(d);
+  S.buf.clear();
+}
+}  // namespace
+
+int ice_gpu_set_dispatch(ice_gpu_ctx* c, int dispatch) {
+  if (dispatch != 0 && dispatch != 1) return fail(c, "[ICE::set_dispatch] dispatch is 0 (queues) or 1 (items)");
+  c->dispatch = dispatch;
+  batched_destroy(c);  // its captured graphs hold the old launches
+  return 0;
+}
+
+int ice_gpu_autotune(ice_gpu_ctx* c, int steps, int32_t* best, double* ms_out) {
+  if (c->rec.empty()) return fail(c, "[ICE::autotune] no boundary table: call ice_gpu_set_boundaries first");
+  if (steps < 1) steps = 1;
+  int ndim = 1;
+  for (const BlkDev& B : c->blk) ndim = std::max(ndim, (B.nj > 1) + (B.nk > 1) + 1);
+  const int(*cand)[3] = ndim == 3 ? kTiles3 : (ndim == 2 ? kTiles2 : kTiles1);
+  const int ncand = ndim == 3 ? 8 : (ndim == 2 ? 6 : 3);
+  int keep[3] = {c->tile[0], c->tile[1], c->tile[2]};
+  Snapshot S;
+  if (snapshot(c, S, true)) {
+    snapshot_free(S);
+    return 1;
+  }
+  double best_ms = std::numeric_limits<double>::infinity();
+  int bi = -1, bd = c->dispatch;
+  std::string err;
+  const int ndisp = c->cfg.mode == ICE_MODE_BATCHED ? 2 : 1;
+  for (int k = 0; k < ncand * ndisp; ++k) {
+    double ms = std::numeric_limits<double>::infinity();
+    if (ndisp == 2) c->dispatch = k / ncand;
+    if (ice_gpu_set_tiles(c, cand[k % ncand][0], cand[k % ncand][1], cand[k % ncand][2]) == 0) {
+      ice_status_t st;
+      // one untimed step (plan, graph capture), then the timed ones
+      int rc = ice_gpu_step(c, 1, &st);
+      if (rc == 0) rc = ice_gpu_step(c, steps, &st);
+      if (rc == 0) ms = c->step_ms / steps;
+      else err = c->err;
+      if (snapshot(c, S, false)) {
+        snapshot_free(S);
+        return 1;
+      }
+    }
+    if (ms_out) ms_out[k] = ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      bi = k % ncand;
+      bd = c->dispatch;
+    }
+  }
+  c->dispatch = bd;
+  snapshot_free(S);
+  if (bi < 0) {
+    c->tile[0] = keep[0]; c->tile[1] = keep[1]; c->tile[2] = keep[2];
+    batched_destroy(c);
+    return fail(c, "[ICE::autotune] no candidate tile ran: %s", err.c_str());
+  }
+  ice_gpu_set_tiles(c, cand[bi][0], cand[bi][1], cand[bi][2]);
+  if (best) {
+    best[0] = cand[bi][0]; best[1] = cand[bi][1]; best[2] = cand[bi][2];
+    best[3] = bd;
+  }
   return 0;
 }
 
@@ -1103,6 +1231,8 @@ BatchLaunch group_launch(ice_gpu_ctx* c, const BatchGroup& g, const std::vector<
   B.Q.ticket = g.d_ticket;
   B.Q.nq = g.nq;
   B.grid = g.grid;
+  B.dispatch = c->dispatch;
+  B.nitems = g.nitems;
   B.riemann = g.riemann;
   B.sd = c->cfg.shock_detector;
   B.coupled = c->cfg.coupled;

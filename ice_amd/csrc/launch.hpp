@@ -119,6 +119,8 @@ struct BatchLaunch {
   const StageLaunch* per_fam;  // record lists of each family of the group
   WorkQueues Q;
   int grid;                    // persistent workgroups
+  int dispatch;                // 0: persistent workgroups over the queues, 1: one workgroup per item
+  int nitems;
   int riemann, sd, coupled, limiter;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // recorded around the persistent stage kernel when set
 };
@@ -154,10 +156,18 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
   }
   reset_tickets(B);
   if (B.ev0) (void)hipEventRecord(B.ev0, B.stream);
-  dim3 grid(B.grid), blk(kThreads);
-  if (B.limiter == 1) hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, 1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
-  else if (B.limiter == 3) hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q);
-  else hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, -1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
+  dim3 blk(kThreads);
+  if (B.dispatch == 1) {
+    dim3 grid((unsigned)B.nitems);
+    if (B.limiter == 1) hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, false, 1>), grid, blk, 0, B.stream, B.d_fams, B.Q.items);
+    else if (B.limiter == 3) hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, false, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q.items);
+    else hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, false, -1>), grid, blk, 0, B.stream, B.d_fams, B.Q.items);
+  } else {
+    dim3 grid(B.grid);
+    if (B.limiter == 1) hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, 1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
+    else if (B.limiter == 3) hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q);
+    else hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, -1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
+  }
   if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
 }
 

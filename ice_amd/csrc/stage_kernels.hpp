@@ -6,6 +6,13 @@
 namespace ice {
 
 constexpr int kThreads = 256;
+// Waves per SIMD the stage kernels of a closure are compiled for, i.e. their
+// register budget (512 VGPRs per SIMD lane shared by the resident waves).  Two
+// waves hide the latency of the face passes' loads and barriers for MK and IG,
+// whose stencils fit 256 registers; AG's 12-slot stencil spills at that budget
+// and runs faster alone.
+template <int M> struct StageWaves { static constexpr int value = 2; };
+template <> struct StageWaves<2> { static constexpr int value = 1; };
 
 // ------------------------------------------------------------ learned seed
 // LDS of the seed per workgroup: per wave 64x16 inputs, 64x64 hidden (BF16)
@@ -526,7 +533,7 @@ ICE_DEV void update_cell(const FamArgs& a, const BlkDev& B, int i, int j, int k,
 // (Mod_Fluxes.f90:471-508), source, residual (Lib_Residual.f90:147-156) and
 // either the RK update (WRITE_R = false) or the residual for smoothing.
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R>
-__global__ void __launch_bounds__(kThreads) k_stage(FamArgs a, int b) {
+__global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage(FamArgs a, int b) {
   constexpr int NC = Closure<M>::NC;
   const BlkDev& B = a.blk[b];
   const BlkDev Bv = B;
@@ -772,7 +779,7 @@ ICE_DEV int xcd_tile(int w, int n) {
 }
 
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
-__global__ void __launch_bounds__(kThreads) k_stage_tiled(FamArgs a, int b, Tile T) {
+__global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_tiled(FamArgs a, int b, Tile T) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
   __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
@@ -805,13 +812,15 @@ ICE_DEV int next_item(const WorkQueues& Q, int q, int* shared_slot) {
     *shared_slot = it < Q.qstart[q + 1] ? (int)it : -1;
   }
   __syncthreads();
-  int it = *shared_slot;
+  // the item is the workgroup's: saying so keeps its fields, and the family's
+  // arguments they index, in scalar registers
+  int it = __builtin_amdgcn_readfirstlane(*shared_slot);
   __syncthreads();
   return it;
 }
 
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
-__global__ void __launch_bounds__(kThreads) k_stage_persist(const FamArgs* fams, WorkQueues Q) {
+__global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_persist(const FamArgs* fams, WorkQueues Q) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
   __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
@@ -823,6 +832,21 @@ __global__ void __launch_bounds__(kThreads) k_stage_persist(const FamArgs* fams,
     const WorkItem w = Q.items[it];
     stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
   }
+}
+
+// The batched stage with one workgroup per work item instead of persistent
+// workgroups: the hardware dispatcher balances the items, dealt to the XCDs in
+// contiguous runs (an item's tile neighbours and its sibling families share
+// that XCD's L2).  Same tiles, same bits as k_stage_persist.
+template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
+__global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_items(const FamArgs* fams,
+                                                                                 const WorkItem* items) {
+  constexpr int NC = Closure<M>::NC;
+  __shared__ double fl[NC * kMaxTileFaces];
+  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
+  const int it = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x));
+  const WorkItem w = items[it];
+  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
 }
 
 // the sensor weights and the dt pass over the same queues, one cell per thread

@@ -43,7 +43,7 @@ module ice_gpu_m
 
   public :: ice_gpu_create, ice_gpu_set_geometry, ice_gpu_set_boundaries, ice_gpu_put_state, ice_gpu_get_state, &
             ice_gpu_put_gas, ice_gpu_put_tau, ice_gpu_step, ice_gpu_residual_norm, ice_gpu_timing, &
-            ice_gpu_last_error, ice_gpu_destroy, ice_gpu_error_text
+            ice_gpu_last_error, ice_gpu_destroy, ice_gpu_error_text, ice_gpu_autotune
 
   interface
     integer(c_int) function ice_gpu_create(device, cfg, fams, nb, dims, ctx) bind(C, name='ice_gpu_create')
@@ -109,6 +109,13 @@ module ice_gpu_m
       type(c_ptr), value :: ctx
       real(c_double) :: step_ms, kernel_ms
       integer(c_int64_t) :: launches
+    end function
+    integer(c_int) function ice_gpu_autotune(ctx, steps, best, ms) bind(C, name='ice_gpu_autotune')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: steps
+      integer(c_int32_t) :: best(4)
+      real(c_double) :: ms(16)
     end function
     integer(c_int) function ice_gpu_last_error(ctx, buf, len) bind(C, name='ice_gpu_last_error')
       import :: c_int, c_char, c_ptr

@@ -25,18 +25,19 @@ module ice_host_m
 
   !> The registry: every key the path reads, with its section and default
   !> (Register_Numerics.f90:28-80, Register_Physics.f90, Read_Ini.f90)
-  integer, parameter :: NKEY = 20
+  integer, parameter :: NKEY = 21
   character(len=24), parameter :: key_sec(NKEY) = [character(len=24) :: &
     'ICE-Parameters', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', &
     'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', &
-    'ICE-Numerics', 'ICE-Physics', 'ICE-Physics', 'ICE-IO', 'ICE-IO', 'ICE-IO', 'ICE-Device', 'ICE-Family']
+    'ICE-Numerics', 'ICE-Physics', 'ICE-Physics', 'ICE-IO', 'ICE-IO', 'ICE-IO', 'ICE-Device', 'ICE-Family', &
+    'ICE-Device']
   character(len=24), parameter :: key_name(NKEY) = [character(len=24) :: &
     'iter-threshold', 'time-scheme', 'cfl', 'dt-max', 'tau-factor', 'cfl-rise-threshold', 'time-accurate', &
     'irs', 'irs-beta', 'space-reconstruction', 'flux-limiter', 'shock-detector', 'riemann-solver', &
-    'drag', 'heat-transfer', 'ic-format', 'sol-format', 'shell-diter', 'mode', 'closure']
+    'drag', 'heat-transfer', 'ic-format', 'sol-format', 'shell-diter', 'mode', 'closure', 'autotune']
   character(len=24), parameter :: key_default(NKEY) = [character(len=24) :: &
     '1000000000', 'RK2', '0.5', '1e-4', '1.0', '0', '.true.', '.false.', '0.5', 'first-order', 'none', 'none', &
-    '', 'NoDrag', 'NoHeat', 'tecplot ascii', 'tecplot ascii', '10', 'plain', 'IG']
+    '', 'NoDrag', 'NoHeat', 'tecplot ascii', 'tecplot ascii', '10', 'plain', 'IG', '.false.']
 
   type, public :: registry_t
     character(len=64) :: value(NKEY)

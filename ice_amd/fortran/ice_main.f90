@@ -19,7 +19,8 @@ program ice_driver
   type(ice_status_t) :: st
   integer(c_int32_t), allocatable :: reci(:,:), donor(:,:), dims(:)
   real(R8), allocatable :: recd(:,:), weight(:), fam(:)
-  real(R8) :: rho_mat, cs, total(5), step_ms, kernel_ms
+  real(R8) :: rho_mat, cs, total(5), step_ms, kernel_ms, tile_ms(16)
+  integer(c_int32_t) :: tile(4)
   integer(c_int64_t) :: launches
   type(c_ptr) :: ctx
   integer :: device, nb, b, nc, ndir, iters, it, n, shell, rc, ndonor
@@ -69,6 +70,10 @@ program ice_driver
     call check(ice_gpu_put_state(ctx, b, 1, blocks(b)%prim), 'put_state')
   end do
 
+  if (reg_bool(reg, 'autotune')) then
+    call check(ice_gpu_autotune(ctx, 2, tile, tile_ms), 'autotune')
+    write(output_unit, '(A,3(1X,I0),A,I0)') ' autotuned stage tile:', tile(1:3), ', dispatch ', tile(4)
+  end if
   iters = reg_int(reg, 'iter-threshold')
   shell = max(1, reg_int(reg, 'shell-diter'))
   allocate(fam(5 * cfg%ngroups))

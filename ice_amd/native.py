@@ -79,6 +79,8 @@ SIGNATURES = {
     "ice_gpu_timing": [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "ice_gpu_set_kernel_timing": [C.c_void_p, C.c_int],
     "ice_gpu_set_tiles": [C.c_void_p, C.c_int, C.c_int, C.c_int],
+    "ice_gpu_set_dispatch": [C.c_void_p, C.c_int],
+    "ice_gpu_autotune": [C.c_void_p, C.c_int, _IP, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")],
     "ice_gpu_mlp_grad": [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP, _FP],
     "ice_gpu_mlp_forward": [C.c_void_p, C.c_int, _FP, _FP, _FP],
     "ice_gpu_surrogate_train": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _FP],

@@ -96,7 +96,7 @@ def test_driver_refuses_irs_time_accurate(tmp_path):
 def test_driver_on_gpu(tmp_path, name, mode):
     print(f"case fortran driver {name} {mode}", flush=True)
     d = str(tmp_path / "case")
-    ref = write_case(d, name, mode)
+    ref = write_case(d, name, mode, extra={"ICE-Device": {"autotune": ".true."}} if name == "IG-split4" else None)
     p = run_driver(d, {}, timeout=300)
     print(p.stdout[-400:], p.stderr[-400:])
     assert p.returncode == 0

@@ -160,3 +160,41 @@ def test_multiblock_irs_steady(gpu, oracle):
     for b in range(1, len(case.blocks) + 1):
         for p in range(1, case.cfg.ngroups + 1):
             assert np.array_equal(rg.state(b, p), ro.state(b, p)), (b, p)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_autotune_keeps_state_and_bits(gpu, mode):
+    """The autotuner's trial steps leave the run where it was, and the tile it
+    picks gives the same bits as the default one."""
+    import ctypes as C
+    case = cases.case_multiblock(nblocks=6, emin=8, emax=20, steps=5, mode=mode)
+    print("case multiblock 6 blocks autotune mode", mode, flush=True)
+    ra = cases.Runner(gpu, case, "ice_gpu_")
+    rb = cases.Runner(gpu, case, "ice_gpu_")
+    ra.step(1)
+    rb.step(1)
+    best = np.zeros(4, np.int32)
+    ms = np.zeros(16)
+    assert gpu.ice_gpu_autotune(ra.ctx, 2, best, ms) == 0, ra.error()
+    print("tile", best.tolist(), "ms/step", ms.tolist())
+    assert best[0] > 0 and np.isfinite(ms).any()
+    sa = ra.step(4)
+    sb = rb.step(4)
+    assert sa.iter == sb.iter == 5 and sa.time == sb.time
+    for b in range(1, len(case.blocks) + 1):
+        for p in range(1, case.cfg.ngroups + 1):
+            assert np.array_equal(ra.state(b, p), rb.state(b, p))
+
+
+def test_item_dispatch_equals_queues(gpu):
+    """The batched stage as one workgroup per item gives the queues' bits."""
+    case = cases.case_multiblock(nblocks=6, emin=8, emax=20, steps=4, mode=1)
+    print("case multiblock 6 blocks dispatch", flush=True)
+    ra = cases.Runner(gpu, case, "ice_gpu_")
+    rb = cases.Runner(gpu, case, "ice_gpu_")
+    assert gpu.ice_gpu_set_dispatch(rb.ctx, 1) == 0
+    ra.step(4)
+    rb.step(4)
+    for b in range(1, len(case.blocks) + 1):
+        for p in range(1, case.cfg.ngroups + 1):
+            assert np.array_equal(ra.state(b, p), rb.state(b, p))
