@@ -45,6 +45,7 @@ WORKLOADS = {
     "riemann1d": lambda mode: cases.case_riemann_1d(n=2000, steps=1000, mode=mode),
     "multiblock3d": lambda mode: cases.case_multiblock(mode=mode),
     "evap3d": lambda mode: cases.case_evaporation(n=256, steps=50, mode=mode),
+    "combined": lambda mode: cases.case_combined(ntree=88, npairs=4, emin=24, emax=96, pair_n=24, mode=mode),
 }
 DESCR = {
     "jets2d": "2-D crossing jets, 1024x1024 cells, AG closure, HLLE, MUSCL van Leer + Jameson sensor, "
@@ -55,6 +56,9 @@ DESCR = {
     "evap3d": "3-D evaporation-heavy block, 256^3 cells, MK closure, Saurel, MUSCL van Leer, drag Schiller-Naumann + "
               "Ranz-Marshall heat + grey-body radiation + CEM evaporation with Langmuir-Knudsen interface and Stefan "
               "blowing, per-cell Yinf 0-0.3, gas 300-2000 K / 1-5 bar, SSP-RK3, FP64",
+    "combined": "combined configuration scaled to 10.2M cells: 88-block connection tree (edges log-uniform 24-96) + "
+                "4 chimera pairs, AG x 2 families, HLLE, full interphase physics with CEM/Langmuir-Knudsen evaporation "
+                "and Stefan blowing, gas as the evaporation case, steady SSP-RK3 with IRS (beta 0.5), FP64",
 }
 
 
@@ -146,6 +150,9 @@ def main():
         for p in range(1, case.cfg.ngroups + 1):
             if gpu.ice_gpu_surrogate_train(run.ctx, p, 200, 512, 3e-3, 7, rep) != 0:
                 raise RuntimeError(run.error())
+        if gpu.ice_gpu_surrogate_retrain_every(run.ctx, 200, 20) != 0:
+            raise RuntimeError(run.error())
+        surrogate["retrain_every_steps"] = 200
         surrogate["training"] = {"samples": int(rep[0]), "loss_initial": float(rep[1]), "loss_final": float(rep[2]),
                                  "rate_ratio_abs_err_mean": float(rep[3]), "rate_ratio_abs_err_max": float(rep[4]),
                                  "train_ms": float(rep[5])}

@@ -738,3 +738,115 @@ def case_from_inputs(name: str, ini: dict, nodes: List[np.ndarray], ic: List[np.
         prim = padded_state(g["dims"], nc, lambda I, J, K, c=c: c[:, I - 1, J - 1, K - 1])
         blocks.append(Block(g["dims"], g, [prim]))
     return Case(name, cfg, [fam], blocks, reci, recd, donor, weight, steps)
+
+
+# ---------------------------------------------------------------- combined configuration
+def case_combined(ntree: int = 88, npairs: int = 4, emin: int = 32, emax: int = 192, pair_n: int = 24,
+                  seed: int = 20261208, steps: int = 10, mode: int = 0, dx: float = 1e-4) -> Case:
+    """Combined: a seeded random tree of `ntree` face-matching blocks (edges
+    log-uniform on [emin, emax], connections 101) plus `npairs` chimera overset
+    block pairs (102 donor lists with linear weights, as case_chimera), AG closure
+    with 2 families (HLLE, MUSCL van Leer), full interphase physics: drag
+    Schiller-Naumann, Ranz-Marshall heat, grey-body radiation, CEM evaporation with
+    the Langmuir-Knudsen interface and Stefan blowing.  Gas fields as the
+    evaporation case: seeded smooth random T 300-2000 K, p 1-5 bar, far-field
+    vapour fraction 0-0.3, |u| <= 10 m/s.  Steady SSP-RK3 with implicit residual
+    smoothing (beta 0.5).  Uniform Cartesian cells of side dx (closed-form metrics).
+    ntree = 88, npairs = 4, emin = 32, emax = 192 is the full configuration (96
+    blocks, ~150 M cells); smaller arguments give the same physics at test sizes."""
+    rng = np.random.default_rng(seed)
+    dims, orig, joins = block_tree(ntree, emin, emax, rng)
+    rho_mat, ng = 1000.0, 2
+    lo = np.min(np.asarray(orig), axis=0)
+    hi = np.max(np.asarray(orig) + np.asarray(dims), axis=0)
+    span = (hi - lo).astype(float)
+    fseed = int(rng.integers(1 << 30))
+    R, gam = 287.05, 1.4
+    medians = [20e-6, 60e-6]
+    blocks = []
+    for b, (dd, o) in enumerate(zip(dims, orig)):
+        geo = geometry.uniform_metrics(dd, dx)
+        c = [((o[m] - lo[m]) + np.arange(dd[m]) + 0.5) / span[m] for m in range(3)]
+        frng = np.random.default_rng(fseed)   # the same fields everywhere, sampled at the block's cells
+
+        def field(frng=frng, c=c):
+            f = geometry.smooth_field_axes(frng, c[0], c[1], c[2], modes=8, kmax=3)
+            return 0.5 + 0.5 * np.tanh(f / 2.0)   # in (0, 1)
+
+        u, v, w = (2.0 * field() - 1.0 for _ in range(3))
+        Tg = 300.0 + 1700.0 * field()
+        p = 1e5 + 4e5 * field()
+        yinf = 0.3 * field()
+        Td = 280.0 + 70.0 * field()
+        rr = field()
+        g5 = np.stack([p / (R * Tg), 10.0 * u / np.sqrt(3.0), 10.0 * v / np.sqrt(3.0), 10.0 * w / np.sqrt(3.0), Tg])
+        gas = gas_arrays(g5, R, gam, 0.026 * (Tg / 300.0) ** 0.7, 1.8e-5 * (Tg / 300.0) ** 0.7)
+        prims = []
+        for q in range(ng):
+            r = np.clip(medians[q] * np.exp(0.6 * (2.0 * rr - 1.0) * 2.0), 1e-6, 100e-6)
+
+            def fill(I, J, K, r=r):
+                rho = 1e-2 * (0.7 + 0.6 * rr)
+                num = rho / (rho_mat * 4.0 / 3.0 * PI * r ** 3)
+                z = np.zeros(I.shape)
+                return np.stack([rho, 0.9 * g5[1], 0.9 * g5[2], 0.9 * g5[3], rho, z, z, rho, z, rho, Td, num])
+
+            prims.append(padded_state(dd, 12, fill))
+        blocks.append(Block(dd, geo, prims, gas, yinf=np.ascontiguousarray(yinf.ravel(order="F"))))
+    partner_face = {}
+    for a, fa, b, fb in joins:
+        partner_face[(a, fa)] = (b, fb)
+        partner_face[(b, fb)] = (a, fa)
+    tb = TableBuilder(ng)
+    for b, dd in enumerate(dims):
+        def codes(f, i, j, k, b=b):
+            return np.array([101 if (b, int(ff)) in partner_face else 400 for ff in f], np.int32)
+
+        def ints(ri, fijk, b=b):
+            for f in range(1, 7):
+                if (b, f) not in partner_face:
+                    continue
+                ob, of = partner_face[(b, f)]
+                m = fijk[:, 0] == f
+                d = (f - 1) // 2
+                src = fijk[m, 1:4].copy()
+                src[:, d] = 1 if of % 2 == 1 else dims[ob][d]
+                ri[m, R_BS] = ob + 1
+                ri[m, R_IS], ri[m, R_JS], ri[m, R_KS] = src[:, 0], src[:, 1], src[:, 2]
+                ri[m, R_FS] = of
+
+        tb.add_block(b + 1, dd, codes, ints_fn=ints)
+    reci, recd, donor, weight = tb.arrays()
+    # the chimera pairs, appended block by block after the tree
+    ri_all, rd_all, dn_all, w_all = [reci], [recd], [donor], [weight]
+    nb = len(blocks)
+    nd = donor.shape[0]
+    for q in range(npairs):
+        pc = case_chimera(n=pair_n, na=2 * pair_n, shift=3.7, closure=AG, ngroups=ng, irs=True, evap=2,
+                          seed=seed + 1 + q, mode=mode)
+        ri = pc.reci.copy()
+        ri[:, R_B] += nb
+        has = ri[:, R_BS] > 0
+        ri[has, R_BS] += nb
+        chim = ri[:, R_TYPE] == 102
+        ri[chim, R_DONOR0] += nd
+        dn = pc.donor.copy()
+        dn[:, 0] += nb
+        ri_all.append(ri)
+        rd_all.append(pc.recd)
+        dn_all.append(dn)
+        w_all.append(pc.weight)
+        blocks.extend(pc.blocks)
+        nb += len(pc.blocks)
+        nd += dn.shape[0]
+    reci = np.ascontiguousarray(np.concatenate(ri_all))
+    recd = np.ascontiguousarray(np.concatenate(rd_all))
+    donor = np.ascontiguousarray(np.concatenate(dn_all).astype(np.int32))
+    weight = np.ascontiguousarray(np.concatenate(w_all))
+    cfg = make_config(ngroups=ng, nrk=3, time_accurate=False, ndir=3, limiter=3, coupled=True, drag_select=4,
+                      heat_select=6, blow_select=1, var_yinf=True, irs=True, irs_beta=0.5, cfl=0.5, dt_max=2e-6,
+                      tau_factor=0.5, mode=mode)
+    fams = [make_family(AG, HLLE, rho_mat=rho_mat, cs=4180.0, lv=2.26e6, emiss=0.9, evap=2, intf=1,
+                        ep=water_ep(alphae=0.04)) for _ in range(ng)]
+    return Case("combined", cfg, fams, blocks, reci, recd, donor, weight, steps,
+                meta={"joins": joins, "tree_blocks": ntree, "pairs": npairs})

@@ -40,6 +40,9 @@ struct SurrDev {
   uint16_t* w = nullptr;
   unsigned long long* stats = nullptr;
   int nparts = 0, on = 0;
+  int batch = 512;                 // training settings of the last training, for retraining
+  double lr = 3e-3;
+  uint64_t seed = 7;
 };
 
 // host view of a record and its pairing (ICE_bc_type)
@@ -79,6 +82,7 @@ struct ice_gpu_ctx {
   BatchedPlan* batched = nullptr;
   std::vector<SurrDev> surr;   // per family
   int lk_stats_on = 0;
+  int retrain_every = 0, retrain_epochs = 20;  // online retraining of the trained networks
   int tile[3] = {-1, -1, -1};  // cells per stage tile; -1 default shape, 0 per-cell kernel
   int dispatch = 0;            // batched stage: 0 persistent queues, 1 one workgroup per item
   std::string err;
@@ -469,6 +473,9 @@ int plain_step(ice_gpu_ctx* c) {
 
 }  // namespace
 
+// online retraining of every trained network on the current state (surrogate.hpp)
+int retrain_networks(ice_gpu_ctx* c);
+
 // ================================================================= C-ABI
 extern "C" {
 
@@ -797,6 +804,7 @@ int ice_gpu_step(ice_gpu_ctx* c, int nsteps, ice_status_t* st) {
   for (int s = 0; s < nsteps; ++s) {
     int rc = (c->cfg.mode == ICE_MODE_BATCHED) ? batched_step(c) : plain_step(c);
     if (rc) return rc;
+    if (c->retrain_every > 0 && c->iter % c->retrain_every == 0 && retrain_networks(c)) return 1;
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1233,6 +1241,10 @@ BatchLaunch group_launch(ice_gpu_ctx* c, const BatchGroup& g, const std::vector<
   B.grid = g.grid;
   B.dispatch = c->dispatch;
   B.nitems = g.nitems;
+  B.irs = c->cfg.irs;
+  B.ndir = c->cfg.ndir;
+  B.nb = (int)c->blk.size();
+  B.blk = c->blk.data();
   B.riemann = g.riemann;
   B.sd = c->cfg.shock_detector;
   B.coupled = c->cfg.coupled;
@@ -1274,7 +1286,6 @@ void launch_batched_step(ice_gpu_ctx* c, const StepArgs& A, bool timed) {
 }  // namespace
 
 int batched_step(ice_gpu_ctx* c) {
-  if (c->cfg.irs) return fail(c, "[ICE::step] the batched mode does not smooth residuals (irs); use the plain mode");
   if (!c->batched && build_plan(c)) return 1;
   BatchedPlan* P = c->batched;
   c->iter += 1;

@@ -121,6 +121,8 @@ struct BatchLaunch {
   int grid;                    // persistent workgroups
   int dispatch;                // 0: persistent workgroups over the queues, 1: one workgroup per item
   int nitems;
+  int irs, ndir, nb;           // implicit residual smoothing: the stage writes R, then per block sweeps
+  const BlkDev* blk;           // host copies of the blocks
   int riemann, sd, coupled, limiter;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // recorded around the persistent stage kernel when set
 };
@@ -157,6 +159,26 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
   reset_tickets(B);
   if (B.ev0) (void)hipEventRecord(B.ev0, B.stream);
   dim3 blk(kThreads);
+  if (B.irs) {
+    // residual into R, smoothed per block and direction, then the update
+    if (B.dispatch == 1)
+      hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, true, -1>), dim3((unsigned)B.nitems), blk, 0, B.stream, B.d_fams,
+                         B.Q.items);
+    else
+      hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, true, -1>), dim3(B.grid), blk, 0, B.stream, B.d_fams, B.Q);
+    if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
+    constexpr int NC = Closure<M>::NC;
+    for (int f = 0; f < B.nfam; ++f) {
+      FamArgs a = B.h_fams[B.fam_ids[f]];
+      for (int d = 0; d < B.ndir; ++d)
+        for (int sw = 1; sw <= 2; ++sw)
+          for (int b = 0; b < B.nb; ++b)
+            hipLaunchKernelGGL(k_irs<NC>, dim3(grid_cells(B.blk[b].cells)), blk, 0, B.stream, a, b, d, sw);
+      for (int b = 0; b < B.nb; ++b)
+        hipLaunchKernelGGL(k_update<M>, dim3(grid_cells(B.blk[b].cells)), blk, 0, B.stream, a, b);
+    }
+    return;
+  }
   if (B.dispatch == 1) {
     dim3 grid((unsigned)B.nitems);
     if (B.limiter == 1) hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, false, 1>), grid, blk, 0, B.stream, B.d_fams, B.Q.items);

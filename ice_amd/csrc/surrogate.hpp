@@ -367,7 +367,11 @@ int ice_gpu_mlp_forward(ice_gpu_ctx* c, int n, const float* x, const float* para
   return 0;
 }
 
-int ice_gpu_surrogate_train(ice_gpu_ctx* c, int p, int epochs, int batch, double lr, uint64_t seed, float* report) {
+// Samples family p's current state with the plain solver and trains its network
+// on them; warm: from its current weights (online retraining), else from fresh
+// He-uniform weights.
+static int surr_train_impl(ice_gpu_ctx* c, int p, int epochs, int batch, double lr, uint64_t seed, float* report,
+                           bool warm) {
   if (p < 1 || p > c->cfg.ngroups) return fail(c, "[ICE::surrogate_train] no family %d", p);
   const ice_family_t& fd = c->fams[p - 1];
   if (fd.evap_select == 0 || fd.intf_select != 1)
@@ -413,11 +417,12 @@ int ice_gpu_surrogate_train(ice_gpu_ctx* c, int p, int epochs, int batch, double
   std::vector<float> P;
   mlp_init(P, seed);
   std::vector<float> zeros(kMlpParams, 0.0f);
-  HIPCHK(hipMemcpy(S.P, P.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  if (!warm || !S.on) HIPCHK(hipMemcpy(S.P, P.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(S.m, zeros.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(S.v, zeros.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
   if (surr_pack(c, S)) return 1;
   float loss0 = 0.f, loss1 = 0.f;
+  batch = std::max(batch, (n + 255) / 256);  // at most 256 optimiser steps per pass over large sample sets
   if (surr_grad(c, S, dX, dY, n, nullptr, 0.f, 0)) return 1;
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(&loss0, S.loss, sizeof(float), hipMemcpyDeviceToHost));
@@ -451,6 +456,9 @@ int ice_gpu_surrogate_train(ice_gpu_ctx* c, int p, int epochs, int batch, double
     mx = std::max(mx, d);
   }
   S.on = 1;
+  S.batch = batch;
+  S.lr = lr;
+  S.seed = seed;
   if (report) {
     report[0] = (float)n;
     report[1] = loss0;
@@ -459,6 +467,17 @@ int ice_gpu_surrogate_train(ice_gpu_ctx* c, int p, int epochs, int batch, double
     report[4] = (float)mx;
     report[5] = ms;
   }
+  return 0;
+}
+
+int ice_gpu_surrogate_train(ice_gpu_ctx* c, int p, int epochs, int batch, double lr, uint64_t seed, float* report) {
+  return surr_train_impl(c, p, epochs, batch, lr, seed, report, false);
+}
+
+int ice_gpu_surrogate_retrain_every(ice_gpu_ctx* c, int every, int epochs) {
+  if (every < 0 || epochs < 1) return fail(c, "[ICE::surrogate_retrain_every] every >= 0 and epochs >= 1");
+  c->retrain_every = every;
+  c->retrain_epochs = epochs;
   return 0;
 }
 
@@ -504,3 +523,12 @@ int ice_gpu_lk_stats(ice_gpu_ctx* c, int p, int64_t* out) {
 }
 
 }  // extern "C"
+
+int retrain_networks(ice_gpu_ctx* c) {
+  for (int p = 1; p <= c->cfg.ngroups; ++p) {
+    SurrDev& S = c->surr[p - 1];
+    if (!S.on) continue;
+    if (surr_train_impl(c, p, c->retrain_epochs, S.batch, S.lr, S.seed, nullptr, true)) return 1;
+  }
+  return 0;
+}

@@ -197,6 +197,10 @@ int  ice_gpu_mlp_forward(ice_gpu_ctx* ctx, int n, const float* x, const float* p
  * error of the rate ratio, milliseconds. */
 int  ice_gpu_surrogate_train(ice_gpu_ctx* ctx, int p, int epochs, int batch, double lr, uint64_t seed,
                              float* report);
+/* Online retraining: after every `every` steps of ice_gpu_step (iteration count a
+ * multiple of it), each trained network is trained again, `epochs` passes from its
+ * current weights, on samples of the current state (0 turns it off). */
+int  ice_gpu_surrogate_retrain_every(ice_gpu_ctx* ctx, int every, int epochs);
 int  ice_gpu_surrogate_params(ice_gpu_ctx* ctx, int p, float* params);
 int  ice_gpu_surrogate_set_params(ice_gpu_ctx* ctx, int p, const float* params);
 /* 1: count the interface solves of every stage; ice_gpu_lk_stats: family p's solves,

@@ -148,15 +148,20 @@ def test_chimera_pair(gpu, oracle, kw):
     assert err <= 1e-12
 
 
-def test_multiblock_irs_steady(gpu, oracle):
+@pytest.mark.parametrize("mode, dispatch", [(0, 0), (1, 0), (1, 1)])
+def test_multiblock_irs_steady(gpu, oracle, mode, dispatch):
     """Steady run (per-cell dt) with implicit residual smoothing over 10 blocks,
-    uncoupled: bit for bit."""
-    case = cases.case_multiblock(nblocks=10, emin=6, emax=14, steps=10)
+    uncoupled, plain and batched (both dispatches): bit for bit."""
+    case = cases.case_multiblock(nblocks=10, emin=6, emax=14, steps=10, mode=mode)
     case.cfg.coupled = 0
     case.cfg.irs = 1
     case.cfg.time_accurate = 0
-    print("case multiblock irs steady", flush=True)
-    rg, ro, sg, so = run_both(gpu, oracle, case, 10)
+    print("case multiblock irs steady mode", mode, "dispatch", dispatch, flush=True)
+    rg = cases.Runner(gpu, case, "ice_gpu_")
+    assert gpu.ice_gpu_set_dispatch(rg.ctx, dispatch) == 0
+    ro = cases.Runner(oracle, case, "orc_")
+    rg.step(10)
+    ro.step(10)
     for b in range(1, len(case.blocks) + 1):
         for p in range(1, case.cfg.ngroups + 1):
             assert np.array_equal(rg.state(b, p), ro.state(b, p)), (b, p)
@@ -198,3 +203,28 @@ def test_item_dispatch_equals_queues(gpu):
     for b in range(1, len(case.blocks) + 1):
         for p in range(1, case.cfg.ngroups + 1):
             assert np.array_equal(ra.state(b, p), rb.state(b, p))
+
+
+def test_combined_batched_irs_evap_chimera(gpu, oracle):
+    """The combined configuration's ingredients in one batched run: a chimera
+    pair, AG with two families, evaporation with the Langmuir-Knudsen interface
+    and Stefan blowing, steady with implicit residual smoothing."""
+    case = cases.case_chimera(closure=2, ngroups=2, irs=True, evap=2, steps=6, mode=1)
+    print("case chimera ag2 irs evap batched", flush=True)
+    rg, ro, sg, so = run_both(gpu, oracle, case, 6)
+    err = compare_states(rg, ro, case)
+    print("rel L2", err)
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_combined_small(gpu, oracle, mode):
+    """The combined configuration at test size: 6 tree blocks and 2 chimera pairs,
+    AG x 2 families, full physics with evaporation, steady with IRS."""
+    case = cases.case_combined(ntree=6, npairs=2, emin=6, emax=12, pair_n=8, steps=3, mode=mode)
+    print("case combined small mode", mode, flush=True)
+    rg, ro, sg, so = run_both(gpu, oracle, case, 3)
+    assert sg.bad_state == so.bad_state == 0
+    err = compare_states(rg, ro, case)
+    print("rel L2", err)
+    assert err <= 1e-12
