@@ -28,16 +28,19 @@ from ice_amd import cases, native  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_kernel_bytes(case) -> float:
-    """Algorithmic HBM bytes per interior cell of one launch of the fused stage
-    kernel: read the stage state and the step's start state, write the new state
-    (3 nc), the geometry the faces and the cell read (1-D 6, 2-D 11, 3-D 16
-    doubles), and the frozen gas (9 doubles) when coupled; per family."""
-    geo = {1: 6, 2: 11, 3: 16}[case.cfg.ndir]
-    per = 0
-    for f in case.fams:
-        per += 3 * native.NCOND[f.closure] + geo + (9 if case.cfg.coupled else 0)
-    return 8.0 * per / len(case.fams)
+def stage_kernel_bytes(case, batched: bool) -> float:
+    """Algorithmic HBM bytes per cell-stage of one family in the fused stage kernel:
+    read the stage state and the step's start state, write the new state (3 nc
+    per family); the geometry the faces and the cell read (1-D 6, 2-D 11, 3-D 16
+    doubles) and the frozen gas (9 doubles) when coupled, once per launch.  A plain
+    launch covers one family; a batched launch all families of the blocks, which
+    share the geometry and gas reads."""
+    geo = {1: 6, 2: 11, 3: 16}[case.cfg.ndir] + (9 if case.cfg.coupled else 0)
+    state = sum(3 * native.NCOND[f.closure] for f in case.fams)
+    nf = len(case.fams)
+    if batched:
+        return 8.0 * (state + geo) / nf
+    return 8.0 * (state / nf + geo)
 
 
 WORKLOADS = {
@@ -183,7 +186,7 @@ def main():
     value = cell_stages / wall / 1e6
     avg_kernel_s = kern_ms.value / max(launches.value, 1) / 1e3
     cells_per_launch = case.cell_stages(args.kernel_steps) / max(launches.value, 1)
-    bytes_per_launch = cells_per_launch * stage_kernel_bytes(case)
+    bytes_per_launch = cells_per_launch * stage_kernel_bytes(case, args.mode == "batched")
     achieved = bytes_per_launch / avg_kernel_s / 1e9 if avg_kernel_s > 0 else None
 
     cpu = None
@@ -215,7 +218,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                      "kernel": "k_stage_tiled / k_stage_persist (fused interior flux + source + residual + RK update)",
-                     "kernel_avg_ms": avg_kernel_s * 1e3, "bytes_per_cell": stage_kernel_bytes(case),
+                     "kernel_avg_ms": avg_kernel_s * 1e3, "bytes_per_cell": stage_kernel_bytes(case, args.mode == "batched"),
                      "launches": launches.value},
         "cpu_baseline": cpu,
     }

@@ -12,7 +12,10 @@ constexpr int kThreads = 256;
 // whose stencils fit 256 registers; AG's 12-slot stencil spills at that budget
 // and runs faster alone.
 template <int M> struct StageWaves { static constexpr int value = 2; };
-template <> struct StageWaves<2> { static constexpr int value = 1; };
+#ifndef ICE_AG_WAVES
+#define ICE_AG_WAVES 1
+#endif
+template <> struct StageWaves<2> { static constexpr int value = ICE_AG_WAVES; };
 
 // ------------------------------------------------------------ learned seed
 // LDS of the seed per workgroup: per wave 64x16 inputs, 64x64 hidden (BF16)
