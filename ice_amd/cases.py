@@ -218,6 +218,64 @@ def case_riemann_1d(n: int = 2000, steps: int = 1000, limiter: int = 1, drag: bo
                 meta={"dx": dx})
 
 
+def case_duct400(closure: int = MK, u: float = -10.0, cells=None, codes=None, inlet=None, tilt: float = 0.0,
+                  gas_u: Optional[float] = None, rho0: Optional[float] = None, steps: int = 20, mode: int = 0) -> Case:
+    """A duct of 16 cells on [0, 0.16] (cross-section 0.01 x 0.01) with extrapolation
+    faces (400) on faces 1-4 and type 0 on 5-6, a uniform cloud moving along x at u
+    in `cells` (1-based, every cell by default; the others empty), 20 time-accurate
+    RK2 steps at CFL 0.5, MUSCL van Leer: the setting of ICE's inflow-400 test
+    (test/fast/inflow-400/run.py:57-140).  MK at 1e-3 kg/m3, IG and AG at 1 kg/m3
+    (IG P = 25; AG P11 = 100, P22 = P33 = 1), T = 300 K, drops of 10 um radius of a
+    1000 kg/m3, 900 J/(kg K) material.  `codes` replaces the face codes, with the
+    402 payload `inlet` (mass flux, speed, temperature, radius, along the normal);
+    `tilt` moves the nodes of i = 0 by -tilt*y along x; `gas_u` adds a gas moving
+    along x that drags the cloud (Stokes, no heat, dt-max 1e-6)."""
+    nx, lx, rad, rho_al, cs = 16, 0.16, 1e-5, 1000.0, 900.0
+    mass = rho_al * 4.0 / 3.0 * PI * rad ** 3
+    dims = (nx, 1, 1)
+    nodes = geometry.cartesian_nodes(np.linspace(0.0, lx, nx + 1), [0.0, lx / nx], [0.0, lx / nx])
+    if tilt:
+        nodes[0, 0] = nodes[0, 0] - tilt * nodes[1, 0]
+    geo = geometry.metrics(nodes)
+    if rho0 is None:
+        rho0 = 1e-3 if closure == MK else 1.0
+    held = np.array([1.0 if cells is None or i in cells else 0.0 for i in range(1, nx + 1)])
+
+    def fill(I, J, K):
+        a = held[I - 1]
+        rho, z = rho0 * a, np.zeros(I.shape)
+        cols = [rho, u * a, z, z]
+        if closure == IG:
+            cols += [25.0 * a]
+        elif closure == AG:
+            cols += [100.0 * a, z, z, a, z, a]
+        return np.stack(cols + [z + 300.0, rho / mass])
+
+    prim = padded_state(dims, NCOND[closure], fill)
+    codes = codes or {1: 400, 2: 400, 3: 400, 4: 400, 5: 0, 6: 0}
+    tb = TableBuilder(1)
+
+    def payload(rd, fijk):
+        if inlet is not None:
+            on = np.isin(fijk[:, 0], [f for f, c in codes.items() if c == 402])
+            rd[on, RD_MASSFLUX], rd[on, RD_VELOCITY], rd[on, RD_TEMPERATURE], rd[on, RD_RADIUS] = inlet
+
+    def normal_dir(ri, fijk):
+        ri[:, R_NORMAL_DIR] = 1
+
+    tb.add_block(1, dims, lambda f, i, j, k: np.vectorize(codes.get)(f), normal_dir, payload)
+    reci, recd, donor, weight = tb.arrays()
+    gas = None
+    if gas_u is not None:
+        g5 = np.zeros((5,) + dims)
+        g5[0], g5[1], g5[4] = 1.2, gas_u, 300.0
+        gas = gas_arrays(g5, 287.05, 1.4, 0.026, 1.8e-5)
+    cfg = make_config(ngroups=1, nrk=2, time_accurate=True, ndir=1, limiter=3, coupled=gas is not None,
+                      drag_select=2, heat_select=8, cfl=0.5, dt_max=1e-6 if gas is not None else 1e-4, mode=mode)
+    fam = make_family(closure, rho_mat=rho_al, cs=cs)
+    return Case("duct400", cfg, [fam], [Block(dims, geo, [prim], gas)], reci, recd, donor, weight, steps)
+
+
 # ---------------------------------------------------------------- case 2
 def fourier_velocity(X, Y, seed: int, modes: int = 16, umax: float = 5.0):
     """Divergence-free-looking 2-D field: sum of `modes` sine modes with wave vectors
@@ -383,6 +441,23 @@ class Runner:
                    "get_bc_maps")
         n = ncell.value
         return partner, owner, cf, first[:n + 1].copy(), rec[:first[n] - 1].copy()
+
+    def inflow400_lines(self):
+        """(lines, dropped): the 400-inflow lines logged since the last call"""
+        from .native import INFLOW400_LOG, Inflow400Line
+        out = (Inflow400Line * INFLOW400_LOG)()
+        n, dropped = self.C.c_int32(), self.C.c_int64()
+        self.check(self.f("inflow400_lines")(self.ctx, out, INFLOW400_LOG, self.C.byref(n), self.C.byref(dropped)),
+                   "inflow400_lines")
+        return [out[k] for k in range(n.value)], dropped.value
+
+    def inflow400_totals(self):
+        from .native import Inflow400Total
+        cap = 6 * len(self.case.blocks) * self.case.cfg.ngroups
+        out = (Inflow400Total * cap)()
+        n = self.C.c_int32()
+        self.check(self.f("inflow400_totals")(self.ctx, out, cap, self.C.byref(n)), "inflow400_totals")
+        return [out[k] for k in range(n.value)]
 
     def close(self):
         if self.ctx:

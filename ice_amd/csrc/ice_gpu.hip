@@ -14,6 +14,7 @@
 
 #include "../../include/ice_gpu.h"
 #include "batched.hpp"
+#include "inflow400.hpp"
 #include "launch.hpp"
 
 using namespace ice;
@@ -31,6 +32,15 @@ struct FamDev {
   long long* bcell_g = nullptr;
   int ngrp = 0, nown = 0, nbcell = 0, nconn = 0;
   int nc = 0;
+};
+
+// The 400 records of a family and its boundaries' accumulators and totals
+// (count_400_inflow); n = 0: the family has no 400 record
+struct I400Fam {
+  int* rec = nullptr;
+  int n = 0;
+  I400Acc* acc = nullptr;
+  I400Tot* tot = nullptr;
 };
 
 // The learned seed of a family: FP32 master weights, Adam moments, BF16
@@ -85,6 +95,9 @@ struct ice_gpu_ctx {
   int retrain_every = 0, retrain_epochs = 20;  // online retraining of the trained networks
   int tile[3] = {-1, -1, -1};  // cells per stage tile; -1 default shape, 0 per-cell kernel
   int dispatch = 0;            // batched stage: 0 persistent queues, 1 one workgroup per item
+  std::vector<I400Fam> i400;   // per family
+  ice_inflow400_line_t* i400_lines = nullptr;  // the line log, ICE_INFLOW400_LOG entries
+  int* i400_n = nullptr;                        // lines logged since the last read
   std::string err;
 };
 
@@ -222,6 +235,7 @@ int build_lists(ice_gpu_ctx* c) {
   int ng = c->cfg.ngroups, nrec = (int)c->rec.size();
   c->bfirst_h.assign(ng, {});
   c->brec_h.assign(ng, {});
+  c->i400.assign(ng, I400Fam());
   for (int p = 1; p <= ng; ++p) {
     FamDev& F = c->fam[p - 1];
     std::vector<int> grp, own, bfirst, brec;
@@ -279,8 +293,37 @@ int build_lists(ice_gpu_ctx* c) {
     std::vector<int> bidx(c->N, -1);
     for (size_t m = 0; m < bg.size(); ++m) bidx[bg[m]] = (int)m;
     HIPCHK(hipMemcpy(F.bidx, bidx.data(), c->N * sizeof(int), hipMemcpyHostToDevice));
+    // the family's 400 records in table order (list_records, Mod_Inflow400.f90:185-200)
+    std::vector<int> r400;
+    for (int n0 : grp)
+      if (c->rec[n0].v[ICE_R_TYPE] == 400) r400.push_back(n0);
+    I400Fam& I = c->i400[p - 1];
+    I.n = (int)r400.size();
+    if (I.n) {
+      size_t nbnd = 6 * c->blk.size();
+      if (dalloc(c, &I.rec, r400.size()) || dalloc(c, &I.acc, nbnd) || dalloc(c, &I.tot, nbnd)) return 1;
+      HIPCHK(hipMemcpy(I.rec, r400.data(), r400.size() * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemset(I.acc, 0, nbnd * sizeof(I400Acc)));
+      HIPCHK(hipMemset(I.tot, 0, nbnd * sizeof(I400Tot)));
+    }
+  }
+  if (!c->i400_n) {
+    if (dalloc(c, &c->i400_lines, ICE_INFLOW400_LOG) || dalloc(c, &c->i400_n, 1)) return 1;
+    HIPCHK(hipMemset(c->i400_n, 0, sizeof(int)));
   }
   return 0;
+}
+
+// count_400_inflow of family p's stage a (Mod_Explicit.f90:129): it reads the
+// stage input's interior boundary cells only, so it may run anywhere in the stage
+// before the input changes roles
+void launch_i400(ice_gpu_ctx* c, int p, const FamArgs& a) {
+  const I400Fam& I = c->i400[p];
+  if (!I.n) return;
+  hipLaunchKernelGGL(k_i400_count, dim3(grid_of(I.n)), dim3(kThreads), 0, c->stream, a, c->fams[p].closure, I.rec, I.n,
+                     I.acc, I.tot);
+  hipLaunchKernelGGL(k_i400_fold, dim3(1), dim3(256), 0, c->stream, (int)(6 * c->blk.size()), p + 1, I.acc, I.tot,
+                     c->i400_lines, c->i400_n, c->d_st);
 }
 
 FamArgs fam_args(ice_gpu_ctx* c, int p) {
@@ -463,6 +506,7 @@ int plain_step(ice_gpu_ctx* c) {
       a.nxt = F.buf[nxt];
       a.srk = srk;
       a.last_stage = srk == c->cfg.nrk;
+      launch_i400(c, p, a);
       dispatch_stage(c, a);
       F.cur = nxt;
     }
@@ -945,7 +989,36 @@ struct Snapshot {
   std::vector<int> cur, old;
   DevStatus st;
   int iter = 0;
+  std::vector<std::vector<I400Tot>> i400;  // per family: the 400 boundaries' totals
+  std::vector<ice_inflow400_line_t> lines;
+  int nlines = 0;
 };
+
+// the 400 counts, totals and line log (the accumulators are zero between steps)
+int snapshot_i400(ice_gpu_ctx* c, Snapshot& S, bool take) {
+  const size_t nbnd = 6 * c->blk.size();
+  if (!c->i400_n) return 0;
+  if (take) {
+    S.i400.assign(c->i400.size(), {});
+    for (size_t p = 0; p < c->i400.size(); ++p)
+      if (c->i400[p].n) {
+        S.i400[p].resize(nbnd);
+        HIPCHK(hipMemcpy(S.i400[p].data(), c->i400[p].tot, nbnd * sizeof(I400Tot), hipMemcpyDeviceToHost));
+      }
+    HIPCHK(hipMemcpy(&S.nlines, c->i400_n, sizeof(int), hipMemcpyDeviceToHost));
+    S.lines.resize(ICE_INFLOW400_LOG);
+    HIPCHK(hipMemcpy(S.lines.data(), c->i400_lines, S.lines.size() * sizeof(ice_inflow400_line_t),
+                     hipMemcpyDeviceToHost));
+    return 0;
+  }
+  for (size_t p = 0; p < c->i400.size(); ++p)
+    if (c->i400[p].n)
+      HIPCHK(hipMemcpy(c->i400[p].tot, S.i400[p].data(), nbnd * sizeof(I400Tot), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->i400_n, &S.nlines, sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->i400_lines, S.lines.data(), S.lines.size() * sizeof(ice_inflow400_line_t),
+                   hipMemcpyHostToDevice));
+  return 0;
+}
 
 int snapshot(ice_gpu_ctx* c, Snapshot& S, bool take) {
   const size_t N = (size_t)c->N;
@@ -968,7 +1041,7 @@ int snapshot(ice_gpu_ctx* c, Snapshot& S, bool take) {
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(&S.st, c->d_st, sizeof S.st, hipMemcpyDeviceToHost));
     S.iter = c->iter;
-    return 0;
+    return snapshot_i400(c, S, true);
   }
   for (size_t p = 0; p < c->fam.size(); ++p) {
     FamDev& F = c->fam[p];
@@ -983,7 +1056,7 @@ int snapshot(ice_gpu_ctx* c, Snapshot& S, bool take) {
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(c->d_st, &S.st, sizeof S.st, hipMemcpyHostToDevice));
   c->iter = S.iter;
-  return 0;
+  return snapshot_i400(c, S, false);
 }
 
 void snapshot_free(Snapshot& S) {
@@ -1059,6 +1132,52 @@ int ice_gpu_set_kernel_timing(ice_gpu_ctx* c, int on) {
   return 0;
 }
 
+int ice_gpu_inflow400_lines(ice_gpu_ctx* c, ice_inflow400_line_t* out, int max, int32_t* n, int64_t* dropped) {
+  if (!c) return 1;
+  *n = 0;
+  *dropped = 0;
+  if (!c->i400_n) return 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int logged = 0;
+  HIPCHK(hipMemcpy(&logged, c->i400_n, sizeof(int), hipMemcpyDeviceToHost));
+  int take = std::min(std::min(logged, ICE_INFLOW400_LOG), std::max(max, 0));
+  if (take > 0) HIPCHK(hipMemcpy(out, c->i400_lines, take * sizeof(ice_inflow400_line_t), hipMemcpyDeviceToHost));
+  *n = take;
+  *dropped = (int64_t)logged - take;
+  int zero = 0;
+  HIPCHK(hipMemcpy(c->i400_n, &zero, sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int ice_gpu_inflow400_totals(ice_gpu_ctx* c, ice_inflow400_total_t* out, int max, int32_t* n) {
+  if (!c) return 1;
+  *n = 0;
+  if (!c->i400_n) return 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t nbnd = 6 * c->blk.size();
+  std::vector<I400Tot> T(nbnd);
+  int k = 0;
+  for (size_t p = 0; p < c->i400.size(); ++p) {
+    if (!c->i400[p].n) continue;
+    HIPCHK(hipMemcpy(T.data(), c->i400[p].tot, nbnd * sizeof(I400Tot), hipMemcpyDeviceToHost));
+    for (size_t m = 0; m < nbnd; ++m) {
+      if (T[m].nstages == 0) continue;
+      if (k >= max) return fail(c, "[ICE::inflow400] more than %d boundaries let particles in", max);
+      ice_inflow400_total_t& o = out[k++];
+      o.block = (int)(m / 6) + 1;
+      o.face = (int)(m % 6) + 1;
+      o.family = (int)p + 1;
+      o.stages = T[m].nstages;
+      o.first_iter = T[m].first_iter;
+      o.first_faces = T[m].first_faces;
+      o.first_largest = T[m].first_x;
+      o.largest = T[m].largest_x;
+    }
+  }
+  *n = k;
+  return 0;
+}
+
 int ice_gpu_last_error(ice_gpu_ctx* c, char* buf, int len) {
   if (!c || len <= 0) return 1;
   std::snprintf(buf, (size_t)len, "%s", c->err.c_str());
@@ -1082,6 +1201,10 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
   for (auto& S : c->surr) {
     F(S.P); F(S.m); F(S.v); F(S.part); F(S.loss); F(S.w); F(S.stats);
   }
+  for (auto& I : c->i400) {
+    F(I.rec); F(I.acc); F(I.tot);
+  }
+  F(c->i400_lines); F(c->i400_n);
   batched_destroy(c);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1268,6 +1391,7 @@ void launch_batched_step(ice_gpu_ctx* c, const StepArgs& A, bool timed) {
     for (auto& g : P->groups) {
       std::vector<StageLaunch> per;
       BatchLaunch B = group_launch(c, g, A.h[srk], A.d[srk], per);
+      for (int p : g.fams) launch_i400(c, p, A.h[srk][p]);
       if (timed) {
         (void)hipEventCreate(&B.ev0);
         (void)hipEventCreate(&B.ev1);

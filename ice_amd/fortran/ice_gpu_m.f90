@@ -41,9 +41,23 @@ module ice_gpu_m
     real(c_double)     :: dtglobal = 0, time = 0
   end type
 
+  !> ice_inflow400_line_t: a line of count_400_inflow
+  type, bind(C), public :: ice_inflow400_line_t
+    integer(c_int32_t) :: iter = 0, block = 0, face = 0, family = 0, faces = 0, relative = 0
+    real(c_double)     :: largest = 0
+  end type
+
+  !> ice_inflow400_total_t: a boundary's totals (report_400_inflow)
+  type, bind(C), public :: ice_inflow400_total_t
+    integer(c_int32_t) :: block = 0, face = 0, family = 0, stages = 0, first_iter = 0, first_faces = 0
+    real(c_double)     :: first_largest = 0, largest = 0
+  end type
+  integer, parameter, public :: ICE_INFLOW400_LOG = 4096
+
   public :: ice_gpu_create, ice_gpu_set_geometry, ice_gpu_set_boundaries, ice_gpu_put_state, ice_gpu_get_state, &
             ice_gpu_put_gas, ice_gpu_put_tau, ice_gpu_step, ice_gpu_residual_norm, ice_gpu_timing, &
-            ice_gpu_last_error, ice_gpu_destroy, ice_gpu_error_text, ice_gpu_autotune
+            ice_gpu_last_error, ice_gpu_destroy, ice_gpu_error_text, ice_gpu_autotune, ice_gpu_inflow400_lines, &
+            ice_gpu_inflow400_totals
 
   interface
     integer(c_int) function ice_gpu_create(device, cfg, fams, nb, dims, ctx) bind(C, name='ice_gpu_create')
@@ -116,6 +130,21 @@ module ice_gpu_m
       integer(c_int), value :: steps
       integer(c_int32_t) :: best(4)
       real(c_double) :: ms(16)
+    end function
+    integer(c_int) function ice_gpu_inflow400_lines(ctx, lines, max, n, dropped) bind(C, name='ice_gpu_inflow400_lines')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, ice_inflow400_line_t
+      type(c_ptr), value :: ctx
+      type(ice_inflow400_line_t) :: lines(*)
+      integer(c_int), value :: max
+      integer(c_int32_t) :: n
+      integer(c_int64_t) :: dropped
+    end function
+    integer(c_int) function ice_gpu_inflow400_totals(ctx, totals, max, n) bind(C, name='ice_gpu_inflow400_totals')
+      import :: c_int, c_int32_t, c_ptr, ice_inflow400_total_t
+      type(c_ptr), value :: ctx
+      type(ice_inflow400_total_t) :: totals(*)
+      integer(c_int), value :: max
+      integer(c_int32_t) :: n
     end function
     integer(c_int) function ice_gpu_last_error(ctx, buf, len) bind(C, name='ice_gpu_last_error')
       import :: c_int, c_char, c_ptr

@@ -90,6 +90,7 @@ program ice_driver
       error stop 1
     end if
     call check(rc, 'step')
+    call print_inflow400_lines()
     it = it + n
     call check(ice_gpu_residual_norm(ctx, total, fam), 'residual_norm')
     write(output_unit, '(A,I8,A,5ES13.5)') ' iter', st%iter, '  res', total
@@ -100,6 +101,8 @@ program ice_driver
     real(cells * cfg%ngroups * cfg%nrk, R8) * iters / (real(t1 - t0, R8) / rate) / 1e6_R8, ' Mcell-stage/s'
   if (st%n_floored > 0) write(output_unit, '(A,I0)') ' [WARNING] floored densities: ', st%n_floored
 
+  call print_inflow400_totals()
+
   do b = 1, nb
     call check(ice_gpu_get_state(ctx, b, 1, blocks(b)%prim), 'get_state')
   end do
@@ -108,6 +111,71 @@ program ice_driver
   call ice_gpu_destroy(ctx)
 
 contains
+
+  !> The 400-inflow lines of the steps just run, in ICE's text (Mod_Inflow400.f90:118-123)
+  subroutine print_inflow400_lines()
+    type(ice_inflow400_line_t), save :: lines(ICE_INFLOW400_LOG)
+    integer(c_int32_t) :: nl
+    integer(c_int64_t) :: dropped
+    integer :: m
+    character(len=48) :: which
+    call check(ice_gpu_inflow400_lines(ctx, lines, ICE_INFLOW400_LOG, nl, dropped), 'inflow400_lines')
+    do m = 1, nl
+      which = ', faces: '
+      if (lines(m)%relative /= 0) which = ', faces above 10 times its last line: '
+      write(output_unit, '(A,I0,A,I0,A)') ' 400 boundary of '//trim(boundary_name(lines(m)%block, lines(m)%face, &
+        lines(m)%family))//': particles entering at iteration ', lines(m)%iter, trim(which)//' ', lines(m)%faces, &
+        ', largest '//trim(measure_text(lines(m)%family, lines(m)%largest))
+    end do
+    if (dropped > 0) write(output_unit, '(A,I0,A)') ' [WARNING] ', dropped, ' lines of the 400 report were dropped'
+    flush(output_unit)
+  end subroutine
+
+  !> The boundaries through which particles entered over the run (Mod_Inflow400.f90:156-169)
+  subroutine print_inflow400_totals()
+    type(ice_inflow400_total_t), allocatable :: t(:)
+    integer(c_int32_t) :: nt
+    integer :: m
+    allocate(t(6 * nb * cfg%ngroups))
+    call check(ice_gpu_inflow400_totals(ctx, t, size(t), nt), 'inflow400_totals')
+    do m = 1, nt
+      write(output_unit, '(A,I0,A,I0,A,I0,A)') ' 400 boundary of '//trim(boundary_name(t(m)%block, t(m)%face, &
+        t(m)%family))//': particles entered in ', t(m)%stages, ' stages, first at iteration ', t(m)%first_iter, &
+        ' (faces: ', t(m)%first_faces, ', largest '//trim(measure_text(t(m)%family, t(m)%first_largest))// &
+        '), largest over the run: '//trim(number_text(t(m)%family, t(m)%largest))
+    end do
+  end subroutine
+
+  function boundary_name(b, f, p) result(s)
+    integer(c_int32_t), intent(in) :: b, f, p
+    character(len=80) :: s
+    write(s, '(A,I0,A,I0)') 'block ', b, ', face ', f
+    if (cfg%ngroups > 1) write(s, '(A,I0)') trim(s)//', family ', p
+  end function
+
+  function measure_text(p, x) result(s)
+    integer(c_int32_t), intent(in) :: p
+    real(R8), intent(in) :: x
+    character(len=60) :: s
+    select case (fams(p)%closure)
+    case (ICE_IG); s = '|w|/sqrt(P/rho): '//number_text(p, x)
+    case (ICE_AG); s = '|w|/sqrt(Pnn/rho): '//number_text(p, x)
+    case default;  s = '|w|: '//number_text(p, x)
+    end select
+  end function
+
+  function number_text(p, x) result(s)
+    integer(c_int32_t), intent(in) :: p
+    real(R8), intent(in) :: x
+    character(len=30) :: s
+    if (x >= 1e100_R8 .or. (x > 0._R8 .and. x < 1e-99_R8)) then
+      write(s, '(ES12.4E3)') x
+    else
+      write(s, '(ES11.4)') x
+    end if
+    s = adjustl(s)
+    if (fams(p)%closure == ICE_MK) s = trim(s)//' m/s'
+  end function
 
   subroutine check(code, what)
     integer(c_int), intent(in) :: code

@@ -55,6 +55,18 @@ class Status(C.Structure):
                 ("bad_recon", C.c_int32), ("iter", C.c_int32), ("dtglobal", C.c_double), ("time", C.c_double)]
 
 
+class Inflow400Line(C.Structure):
+    _fields_ = [("iter", C.c_int32), ("block", C.c_int32), ("face", C.c_int32), ("family", C.c_int32),
+                ("faces", C.c_int32), ("relative", C.c_int32), ("largest", C.c_double)]
+
+
+class Inflow400Total(C.Structure):
+    _fields_ = [("block", C.c_int32), ("face", C.c_int32), ("family", C.c_int32), ("stages", C.c_int32),
+                ("first_iter", C.c_int32), ("first_faces", C.c_int32), ("first_largest", C.c_double),
+                ("largest", C.c_double)]
+
+
+INFLOW400_LOG = 4096
 _DP = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _IP = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _FP = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -89,6 +101,9 @@ SIGNATURES = {
     "ice_gpu_surrogate_set_params": [C.c_void_p, C.c_int, _FP],
     "ice_gpu_set_lk_stats": [C.c_void_p, C.c_int],
     "ice_gpu_lk_stats": [C.c_void_p, C.c_int, _LP],
+    "ice_gpu_inflow400_lines": [C.c_void_p, C.POINTER(Inflow400Line), C.c_int, C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int64)],
+    "ice_gpu_inflow400_totals": [C.c_void_p, C.POINTER(Inflow400Total), C.c_int, C.POINTER(C.c_int32)],
     "ice_gpu_last_error": [C.c_void_p, C.c_char_p, C.c_int],
     "ice_gpu_destroy": [C.c_void_p],
 }

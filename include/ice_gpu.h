@@ -208,6 +208,36 @@ int  ice_gpu_surrogate_set_params(ice_gpu_ctx* ctx, int p, const float* params);
 int  ice_gpu_set_lk_stats(ice_gpu_ctx* ctx, int on);
 int  ice_gpu_lk_stats(ice_gpu_ctx* ctx, int p, int64_t* counts);
 
+/* ---- particles entering through extrapolation faces (400): count_400_inflow and
+ * report_400_inflow (src/lib/diagnostic/Mod_Inflow400.f90:52-171).  Every stage
+ * counts, per boundary (the 400 records of one face of one block, for one family),
+ * the records whose cell carries the phase (rho > rho_empty) and moves into the
+ * domain faster than round-off, and measures it (|w| for MK, |w|/sqrt(P/rho) for IG,
+ * |w|/sqrt(Pnn/rho) for AG).  A line is logged at a boundary's first such stage and
+ * each time a stage's largest measure exceeds 10 times that of its last line.  The
+ * device logs lines in stage order, boundaries by block then face; the host prints
+ * them (the reference's text: ICE prints and flushes them at the stage). */
+typedef struct {
+  int32_t iter;             /* iter_general of the stage                                 */
+  int32_t block, face, family;
+  int32_t faces;            /* records above 10 times the boundary's last line           */
+  int32_t relative;         /* 1: the boundary had a line before ("faces above 10 times") */
+  double  largest;          /* their largest measure                                     */
+} ice_inflow400_line_t;
+/* A boundary's totals over the run (report_400_inflow). */
+typedef struct {
+  int32_t block, face, family, stages, first_iter, first_faces;
+  double  first_largest, largest;
+} ice_inflow400_total_t;
+#define ICE_INFLOW400_LOG 4096   /* lines the device keeps between two reads */
+/* The lines logged since the last call, oldest first: *n = min(logged, max,
+ * ICE_INFLOW400_LOG) lines into out, *dropped = the lines logged beyond them.
+ * Clears the log. */
+int  ice_gpu_inflow400_lines(ice_gpu_ctx* ctx, ice_inflow400_line_t* out, int max, int32_t* n, int64_t* dropped);
+/* The totals of every boundary that let particles in, in the reference's report
+ * order (family, block, face): *n of them, at most max. */
+int  ice_gpu_inflow400_totals(ice_gpu_ctx* ctx, ice_inflow400_total_t* out, int max, int32_t* n);
+
 int  ice_gpu_last_error(ice_gpu_ctx* ctx, char* buf, int len);
 void ice_gpu_destroy(ice_gpu_ctx* ctx);
 

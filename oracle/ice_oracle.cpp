@@ -79,6 +79,11 @@ struct orc_ctx {
   int bad_b = 0, bad_i = 0, bad_j = 0, bad_k = 0, bad_p = 0;
   double restot[5] = {0, 0, 0, 0, 0};
   std::vector<double> resfam;
+  // count_400_inflow (Mod_Inflow400.f90:42-43), per family, 6 per block
+  std::vector<std::vector<int>> i400_nstages, i400_first_iter, i400_first_faces;
+  std::vector<std::vector<double>> i400_first_x, i400_largest_x, i400_line_x;
+  std::vector<ice_inflow400_line_t> i400_lines;
+  int64_t i400_logged = 0;
   std::string err;
 };
 
@@ -240,6 +245,78 @@ const double* bnd_face(const Blk& B, const Rec& r) {
   if (fm <= 2) return FN(B, 0, im - fm % 2, jm, km);
   if (fm <= 4) return FN(B, 1, im, jm - fm % 2, km);
   return FN(B, 2, im, jm, km - fm % 2);
+}
+
+// count_400_inflow (Mod_Inflow400.f90:52-128): one serial pass over family p's
+// 400 records in table order, folded into the boundaries' counts; the lines it
+// would print are logged (at most ICE_INFLOW400_LOG kept between two reads)
+void count_400_inflow(orc_ctx* c, int p) {
+  const int nb = (int)c->blk.size(), model = c->fam[p - 1].model;
+  if (c->i400_nstages.empty()) {
+    int ng = c->cfg.ngroups;
+    c->i400_nstages.assign(ng, std::vector<int>(6 * nb, 0));
+    c->i400_first_iter = c->i400_nstages;
+    c->i400_first_faces = c->i400_nstages;
+    c->i400_first_x.assign(ng, std::vector<double>(6 * nb, 0.0));
+    c->i400_largest_x = c->i400_first_x;
+    c->i400_line_x = c->i400_first_x;
+  }
+  std::vector<int> nface(6 * nb, 0), nabove(6 * nb, 0);
+  std::vector<double> xs(6 * nb, 0.0), xa(6 * nb, 0.0);
+  std::vector<double>& line_x = c->i400_line_x[p - 1];
+  for (int n : c->grp[p - 1]) {
+    const Rec& r = c->bc[n - 1];
+    if (r.type != 400) continue;
+    const Blk& B = c->blk[r.b - 1];
+    const double* q = P(c, r.b, p, r.i, r.j, r.k);
+    if (!(q[0] > RHO_EMPTY)) continue;
+    const double* fn = bnd_face(B, r);
+    double sg = (double)(1 - 2 * (r.f % 2)), normal[3] = {sg * fn[0], sg * fn[1], sg * fn[2]};
+    double w = q[1] * normal[0] + q[2] * normal[1] + q[3] * normal[2];
+    if (!(w < -1e-25)) continue;
+    double x;
+    if (model == ICE_IG) {
+      x = -w * std::sqrt(q[0] / q[4]);
+      if (!(x > 1e-12)) continue;
+    } else if (model == ICE_AG) {
+      double pnn = q[4] * (normal[0] * normal[0]) + q[7] * (normal[1] * normal[1]) + q[9] * (normal[2] * normal[2]) +
+                   2.0 * (q[5] * normal[0] * normal[1] + q[6] * normal[0] * normal[2] + q[8] * normal[1] * normal[2]);
+      x = -w * std::sqrt(q[0] / fmax2(pnn, 2.2250738585072014e-308));
+      if (!(x > 1e-12)) continue;
+    } else {
+      x = -w;
+      if (!(x > 1e-12 * norm2_3(q[1], q[2], q[3]))) continue;
+    }
+    int m = (r.b - 1) * 6 + (r.f - 1);
+    nface[m] += 1;
+    xs[m] = fmax2(xs[m], x);
+    if (x > 10.0 * line_x[m]) {
+      nabove[m] += 1;
+      xa[m] = fmax2(xa[m], x);
+    }
+  }
+  for (int m = 0; m < 6 * nb; ++m) {
+    if (nface[m] == 0) continue;
+    if (c->i400_nstages[p - 1][m] == 0) {
+      c->i400_first_iter[p - 1][m] = c->iter;
+      c->i400_first_faces[p - 1][m] = nface[m];
+      c->i400_first_x[p - 1][m] = xs[m];
+    }
+    c->i400_nstages[p - 1][m] += 1;
+    c->i400_largest_x[p - 1][m] = fmax2(c->i400_largest_x[p - 1][m], xs[m]);
+    if (nabove[m] == 0) continue;
+    ice_inflow400_line_t L;
+    L.iter = c->iter;
+    L.block = m / 6 + 1;
+    L.face = m % 6 + 1;
+    L.family = p;
+    L.faces = nabove[m];
+    L.relative = line_x[m] > 0.0 ? 1 : 0;
+    L.largest = xa[m];
+    if (c->i400_logged < ICE_INFLOW400_LOG) c->i400_lines.push_back(L);
+    c->i400_logged += 1;
+    line_x[m] = xa[m];
+  }
 }
 
 // Lib_Ghost.f90:439-466
@@ -767,6 +844,7 @@ void one_step(orc_ctx* c) {
 #pragma omp parallel for schedule(dynamic, 64)
         for (int ii = 0; ii < (int)g.size(); ++ii) second_ghost_record(c, g[ii]);
       }
+      count_400_inflow(c, p);
       // compute_beta (Mod_Fluxes.f90:71-105, 366-389)
       for (auto& B : c->blk) {
         Phase& ph = B.ph[p - 1];
@@ -1079,6 +1157,40 @@ int orc_get_source(orc_ctx* c, int b, int p, double* source, double* tau) {
 int orc_residual_norm(orc_ctx* c, double* total, double* fam) {
   for (int m = 0; m < 5; ++m) total[m] = c->restot[m];
   for (size_t m = 0; m < c->resfam.size(); ++m) fam[m] = c->resfam[m];
+  return 0;
+}
+
+int orc_inflow400_lines(orc_ctx* c, ice_inflow400_line_t* out, int max, int32_t* n, int64_t* dropped) {
+  int take = (int)std::min<int64_t>((int64_t)c->i400_lines.size(), std::max(max, 0));
+  for (int k = 0; k < take; ++k) out[k] = c->i400_lines[k];
+  *n = take;
+  *dropped = c->i400_logged - take;
+  c->i400_lines.clear();
+  c->i400_logged = 0;
+  return 0;
+}
+
+// report_400_inflow's order (Mod_Inflow400.f90:156-169): family, block, face
+int orc_inflow400_totals(orc_ctx* c, ice_inflow400_total_t* out, int max, int32_t* n) {
+  int k = 0;
+  for (size_t p = 0; p < c->i400_nstages.size(); ++p)
+    for (size_t m = 0; m < c->i400_nstages[p].size(); ++m) {
+      if (c->i400_nstages[p][m] == 0) continue;
+      if (k >= max) {
+        c->err = "[ICE::inflow400] more boundaries let particles in than the output holds";
+        return 1;
+      }
+      ice_inflow400_total_t& o = out[k++];
+      o.block = (int)(m / 6) + 1;
+      o.face = (int)(m % 6) + 1;
+      o.family = (int)p + 1;
+      o.stages = c->i400_nstages[p][m];
+      o.first_iter = c->i400_first_iter[p][m];
+      o.first_faces = c->i400_first_faces[p][m];
+      o.first_largest = c->i400_first_x[p][m];
+      o.largest = c->i400_largest_x[p][m];
+    }
+  *n = k;
   return 0;
 }
 

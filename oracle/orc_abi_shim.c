@@ -20,6 +20,8 @@ int orc_put_tau(orc_ctx*, int, int, const double*);
 int orc_step(orc_ctx*, int, ice_status_t*);
 int orc_get_source(orc_ctx*, int, int, double*, double*);
 int orc_residual_norm(orc_ctx*, double*, double*);
+int orc_inflow400_lines(orc_ctx*, ice_inflow400_line_t*, int, int32_t*, int64_t*);
+int orc_inflow400_totals(orc_ctx*, ice_inflow400_total_t*, int, int32_t*);
 int orc_last_error(orc_ctx*, char*, int);
 void orc_destroy(orc_ctx*);
 
@@ -51,5 +53,11 @@ int ice_gpu_timing(ice_gpu_ctx* x, double* s, double* k, int64_t* n) {
 }
 int ice_gpu_set_tiles(ice_gpu_ctx* x, int a, int b, int c) { (void)x; (void)a; (void)b; (void)c; return 0; }
 int ice_gpu_set_kernel_timing(ice_gpu_ctx* x, int on) { (void)x; (void)on; return 0; }
+int ice_gpu_inflow400_lines(ice_gpu_ctx* x, ice_inflow400_line_t* o, int m, int32_t* n, int64_t* d) {
+  return orc_inflow400_lines(C(x), o, m, n, d);
+}
+int ice_gpu_inflow400_totals(ice_gpu_ctx* x, ice_inflow400_total_t* o, int m, int32_t* n) {
+  return orc_inflow400_totals(C(x), o, m, n);
+}
 int ice_gpu_last_error(ice_gpu_ctx* x, char* buf, int len) { return orc_last_error(C(x), buf, len); }
 void ice_gpu_destroy(ice_gpu_ctx* x) { orc_destroy(C(x)); }
