@@ -276,6 +276,54 @@ def case_duct400(closure: int = MK, u: float = -10.0, cells=None, codes=None, in
     return Case("duct400", cfg, [fam], [Block(dims, geo, [prim], gas)], reci, recd, donor, weight, steps)
 
 
+def case_box(n: int = 16, closure: int = IG, flat: bool = False, seed: int = 20261209, steps: int = 10,
+             mode: int = 0, nodes=None, steady: bool = True) -> Case:
+    """One block of n x n x n cells (flat: n x n x 1) on the unit box, a smooth random
+    cloud (density, velocity, pressure or tensor, temperature), extrapolation (400)
+    faces, steady RK2, MUSCL van Leer: a level of ICE's grid sequencing.  `nodes`
+    gives the block's nodes instead (a coarser level: coarsen())."""
+    rng = np.random.default_rng(seed)
+    if nodes is None:
+        x = np.linspace(0.0, 1.0, n + 1)
+        nodes = geometry.cartesian_nodes(x, x, [0.0, 1.0 / n] if flat else x)
+    geo = geometry.metrics(nodes)
+    dims = geo["dims"]
+    nc = NCOND[closure]
+
+    def fill(I, J, K):
+        X, Y, Z = (I - 0.5) / dims[0], (J - 0.5) / dims[1], (K - 0.5) / dims[2]
+        ph = rng.uniform(0, 2 * PI, 8)
+        s = lambda m: np.sin(2 * PI * X + ph[m]) * np.cos(2 * PI * Y + ph[m + 1])  # noqa: E731
+        rho = 1.0 + 0.3 * s(0)
+        cols = [rho, 2.0 * s(1), 2.0 * s(2), (0.0 if flat else 1.0) * s(3) + 0.0 * Z]
+        if closure == IG:
+            cols += [rho * (1.0 + 0.2 * s(4))]
+        elif closure == AG:
+            z = np.zeros(I.shape)
+            cols += [rho * (1.0 + 0.2 * s(4)), 0.05 * rho * s(5), z, rho * (1.0 + 0.1 * s(6)), z, rho + z]
+        return np.stack(cols + [300.0 + 10.0 * s(2), rho / 1e-12])
+
+    prim = padded_state(dims, nc, fill)
+    tb = TableBuilder(1)
+    tb.add_block(1, dims, lambda f, i, j, k: np.where(f <= (4 if flat else 6), 400, 0))
+    reci, recd, donor, weight = tb.arrays()
+    cfg = make_config(ngroups=1, nrk=2, time_accurate=not steady, ndir=2 if flat else 3, limiter=3, mode=mode)
+    fam = make_family(closure, rho_mat=1000.0, cs=900.0)
+    return Case("box", cfg, [fam], [Block(dims, geo, [prim])], reci, recd, donor, weight, steps,
+                meta={"nodes": nodes, "closure": closure, "flat": flat})
+
+
+def coarsen(case: Case) -> Case:
+    """The next coarser level of a case_box (Setup_Multigrid, Mod_Multigrid.f90:37-70):
+    every other node (a flat block keeps its two k nodes, Coarse_Grid), the same faces."""
+    nodes = case.meta["nodes"]
+    flat = case.meta["flat"]
+    coarse = nodes[:, ::2, ::2, :] if flat else nodes[:, ::2, ::2, ::2]
+    c = case_box(closure=case.meta["closure"], flat=flat, nodes=np.ascontiguousarray(coarse), mode=case.cfg.mode,
+                 steady=not case.cfg.time_accurate)
+    return c
+
+
 # ---------------------------------------------------------------- case 2
 def fourier_velocity(X, Y, seed: int, modes: int = 16, umax: float = 5.0):
     """Divergence-free-looking 2-D field: sum of `modes` sine modes with wave vectors
@@ -382,6 +430,8 @@ class Runner:
                                             np.ascontiguousarray(case.recd), case.donor.shape[0],
                                             np.ascontiguousarray(case.donor.reshape(-1)),
                                             np.ascontiguousarray(case.weight)), "set_boundaries")
+        for p, tab in case.meta.get("tables", {}).items():
+            self.check(self.f("set_properties")(ctx, p, tab.t_min, tab.n, tab.cols, tab.flags), "set_properties")
         for b, B in enumerate(case.blocks, start=1):
             for p in range(1, case.cfg.ngroups + 1):
                 self.check(self.f("put_state")(ctx, b, p, B.prim[p - 1]), "put_state")
@@ -441,6 +491,24 @@ class Runner:
                    "get_bc_maps")
         n = ncell.value
         return partner, owner, cf, first[:n + 1].copy(), rec[:first[n] - 1].copy()
+
+    def set_res_threshold(self, threshold: float):
+        self.check(self.f("set_res_threshold")(self.ctx, threshold), "set_res_threshold")
+
+    def res_stop(self):
+        """(stopped, reference) of the last step call"""
+        s, r = self.C.c_int32(), self.C.c_double()
+        self.check(self.f("res_stop")(self.ctx, self.C.byref(s), self.C.byref(r)), "res_stop")
+        return bool(s.value), r.value
+
+    def source(self, b: int, p: int):
+        """(source (nc, ni, nj, nk), tau (ni, nj, nk)) kept by the last step"""
+        B = self.case.blocks[b - 1]
+        nc = NCOND[self.case.fams[p - 1].closure]
+        ni, nj, nk = B.dims
+        src, tau = np.zeros(nc * ni * nj * nk), np.zeros(ni * nj * nk)
+        self.check(self.f("get_source")(self.ctx, b, p, src, tau), "get_source")
+        return src.reshape((nc, ni, nj, nk), order="F"), tau.reshape((ni, nj, nk), order="F")
 
     def inflow400_lines(self):
         """(lines, dropped): the 400-inflow lines logged since the last call"""
@@ -601,8 +669,25 @@ def water_ep(yinf: float = 0.0, alphae: float = 0.04, cpv: float = 0.0):
     return [Mv, Lv, cpv, 1.0, yinf, Lv * Mv / 8314.46, 1.0 / Tb, alphae]
 
 
+def water_table(t_min: int = 250, t_max: int = 450):
+    """A property table of liquid water on t_min..t_max K for ice_gpu_set_properties,
+    every column varying: cp = 4180 + 0.8 (T - 300) with its exact enthalpy, density
+    1000 - 0.25 (T - 280), Clausius-Clapeyron Psat at the nodes, Lv = 2.5e6 - 2400
+    (T - 273.15), Cpv = 1850 + 0.3 (T - 300), emissivity 0.85 + 5e-4 (T - 250)."""
+    from . import properties
+    T = np.arange(t_min, t_max + 1, dtype=float)
+    cp = 4180.0 + 0.8 * (T - 300.0)
+    h = 4180.0 * T + 0.4 * (T - 300.0) ** 2 - 0.4 * 300.0 ** 2
+    ep = water_ep()
+    psat = 101325.0 * np.exp(-ep[5] * (1.0 / T - ep[6]))
+    return properties.material_table(T, cp, 1000.0 - 0.25 * (T - 280.0), h, psat=psat,
+                                      lv=2.5e6 - 2400.0 * (T - 273.15), cpv=1850.0 + 0.3 * (T - 300.0),
+                                      emiss=0.85 + 5e-4 * (T - 250.0), evaporates=True)
+
+
 def case_evaporation(n: int = 256, steps: int = 50, seed: int = 20261206, mode: int = 0, evap: int = 2,
-                     intf: int = 1, blow: int = 1, box: float = 25.6e-3, limiter: int = 3) -> Case:
+                     intf: int = 1, blow: int = 1, box: float = 25.6e-3, limiter: int = 3,
+                     table: bool = False) -> Case:
     """Evaporation-heavy: one n^3 block of side `box` (25.6 mm at n = 256), MK closure
     (Saurel flux), water drops in a frozen gas with seeded smooth random fields
     (16 modes each): T 300-2000 K, p 1-5 bar (rho_g = p / (R T)), far-field vapour
@@ -656,7 +741,8 @@ def case_evaporation(n: int = 256, steps: int = 50, seed: int = 20261206, mode: 
                       tau_factor=0.5, mode=mode)
     fam = make_family(MK, SAUREL, rho_mat=rho_mat, cs=4180.0, lv=2.26e6, emiss=0.9, evap=evap, intf=intf,
                       ep=water_ep())
-    return Case("evaporation", cfg, [fam], [blk], reci, recd, donor, weight, steps)
+    meta = {"tables": {1: water_table()}} if table else {}
+    return Case("evaporation", cfg, [fam], [blk], reci, recd, donor, weight, steps, meta=meta)
 
 
 # ---------------------------------------------------------------- chimera (overset) pair

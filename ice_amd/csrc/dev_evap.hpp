@@ -20,6 +20,14 @@ constexpr double kXsCap = 1e-12;    // :24
 constexpr double kHuge = 1.7976931348623157e308;
 // ep slots (Lib_Evaporation.f90:15-19), 0-based
 enum { kMv = 0, kLv = 1, kCpv = 2, kLe = 3, kYinf = 4, kLvMvRu = 5, kInvTb = 6, kAlphaE = 7 };
+// A cell's ep carries one more slot: the property table's saturation pressure at
+// the surface temperature (psatExt, Mod_Sources.f90:531-535), NaN for the
+// Clausius-Clapeyron curve (psat_CC, Lib_Evaporation.f90:738-745)
+enum { kPsatX = 8, kNepCell = 9 };
+
+ICE_DEV double surface_psat(const double* ep, double Tp) {
+  return ep[kPsatX] == ep[kPsatX] ? ep[kPsatX] : kPatm * exp(-ep[kLvMvRu] * (1.0 / Tp - ep[kInvTb]));
+}
 
 ICE_DEV double molar2mass_d(double Xs, double Mv, double Mg) { return Xs * Mv / (Xs * Mv + (1.0 - Xs) * Mg); }
 ICE_DEV double mass2molar_d(double Ys, double Mv, double Mg) { return Ys * Mg / (Ys * Mg + (1.0 - Ys) * Mv); }
@@ -209,7 +217,7 @@ ICE_DEV bool evap_surface(GasSide& s, double rhog, double Tg, double gamma, doub
   s.cpg = gamma * Rg / (gamma - 1.0);
   s.p = rhog * Rg * Tg;
   s.Mg = kRu / Rg;
-  double psat = kPatm * exp(-ep[kLvMvRu] * (1.0 / Tp - ep[kInvTb]));
+  double psat = surface_psat(ep, Tp);
   Xs = psat >= s.p ? 1.0 - kXsCap : dmin2(psat / s.p, 1.0 - kXsCap);
   Ys = molar2mass_d(Xs, ep[kMv], s.Mg);
   if (Ys <= ep[kYinf]) return false;
@@ -232,8 +240,11 @@ ICE_DEV double lk_factor(const GasSide& s, const double* ep) {
 // Secant steps from the seed m0 and its Picard image, each kept inside the
 // bracket (else the bracket's midpoint), converged by lkCorrection's own test
 // (the map moves m by at most 1e-12 |map(m)|), whose map evaluation is then
-// the result.  False after kSeedEvals evaluations.
-constexpr int kSeedEvals = 16;
+// the result.  A secant that keeps landing on one side of the root shrinks the
+// bracket slowly: every second step that has not halved the bracket is replaced
+// by a bisection, so the bracket at least halves every three evaluations.  False
+// after kSeedEvals evaluations.
+constexpr int kSeedEvals = 64;
 ICE_DEV bool lk_secant(const GasSide& s, double XsEq, double LKd2, double md_eq, double m0, double& md, double& Qd,
                        bool& ovr, int& evals) {
   double lo = md_eq, hi = 0.0;
@@ -252,6 +263,8 @@ ICE_DEV bool lk_secant(const GasSide& s, double XsEq, double LKd2, double md_eq,
   else hi = m0;
   double m1 = f0;
   if (!(m1 > lo && m1 < hi)) m1 = 0.5 * (lo + hi);
+  double wref = hi - lo;
+  int since = 0;
   while (evals < kSeedEvals) {
     lk_map(s, XsEq, LKd2, m1, f1, Q, o);
     ++evals;
@@ -267,6 +280,11 @@ ICE_DEV bool lk_secant(const GasSide& s, double XsEq, double LKd2, double md_eq,
     double den = g1 - g0;
     double m2 = den != 0.0 ? m1 - g1 * (m1 - m0) / den : 0.5 * (lo + hi);
     if (!(m2 > lo && m2 < hi)) m2 = 0.5 * (lo + hi);
+    if (++since == 2) {
+      if (hi - lo > 0.5 * wref) m2 = 0.5 * (lo + hi);
+      wref = hi - lo;
+      since = 0;
+    }
     m0 = m1;
     g0 = g1;
     m1 = m2;
@@ -332,7 +350,7 @@ ICE_DEV bool lk_needed(double rhog, double Tg, double Rg, double Tp, int evap, i
   p = rhog * Rg * Tg;
   if (evap == 0 || intf != 1) return false;
   double Mg = kRu / Rg;
-  double psat = kPatm * exp(-ep[kLvMvRu] * (1.0 / Tp - ep[kInvTb]));
+  double psat = surface_psat(ep, Tp);
   double Xs = psat >= p ? 1.0 - kXsCap : dmin2(psat / p, 1.0 - kXsCap);
   double Ys = molar2mass_d(Xs, ep[kMv], Mg);
   if (Ys <= ep[kYinf]) return false;

@@ -36,11 +36,25 @@ template <> struct Closure<0> { static constexpr int NC = 6; };
 template <> struct Closure<1> { static constexpr int NC = 7; };
 template <> struct Closure<2> { static constexpr int NC = 12; };
 
-// Material constants of a family (constant properties).
+// Material of a family: constant properties, and optionally a property table
+// on the integer kelvins tmin..tmax (Load_Table.f90:131-147), one column of
+// tmax - tmin + 1 values per property in the kProp* order, with flags saying
+// which tabulated properties the path takes.  rho and cs hold the table's
+// values at tmin when a table is set (mat_rho without rho_varies, mat_cp_const).
+enum : int { kPropRho = 0, kPropCp, kPropE, kPropPsat, kPropLv, kPropCpv, kPropEmiss, kPropCols };
+enum : int { kRhoVaries = 1, kCsVaries = 2, kUsePsat = 4, kUseLv = 8, kUseCpv = 16, kUseEmiss = 32 };
 struct MatDev {
   double rho, cs, lv, emiss;
   double ep[8];
-  int evap, intf, surrogate, pad;
+  int evap, intf, surrogate, flags;
+  const double* tab;  // kPropCols columns; null without a table
+  int tmin, tmax;
+};
+// The material seen by the launches that serve families without tabulated
+// properties: its flags are the constant 0, so every table branch of the functions
+// below, which take the material's type as a template parameter, folds away.
+struct MatConst : MatDev {
+  static constexpr int flags = 0;
 };
 
 // NaN-ignoring two-argument min / max (Fortran MIN, MAX, MINVAL, MAXVAL)
@@ -74,19 +88,81 @@ ICE_DEV double norm3(double a, double b, double c) {
   return scale * sqrt(ssq);
 }
 
+// ------------------------------------------------------------- properties
+// Lib_Properties.f90:163-250.  A column of the table, index T - tmin.
+template <class MT> ICE_DEV const double* prop_col(const MT& m, int c) { return m.tab + (long long)c * (m.tmax - m.tmin + 1); }
+
+// table_value: linear between the nodes, the end value outside, NaN for NaN
+ICE_DEV double table_value(const double* t, int lo, int hi, double T) {
+  if (T != T) return T;
+  double Tc = fmin(fmax(T, (double)lo), (double)hi);
+  if (Tc >= (double)hi) return t[hi - lo];
+  int i = (int)Tc;
+  return t[i - lo] + (t[i + 1 - lo] - t[i - lo]) * (Tc - (double)i);
+}
+template <class MT> ICE_DEV double prop_value(const MT& m, int c, double T) { return table_value(prop_col(m, c), m.tmin, m.tmax, T); }
+
+// mat_e: the energy column between the nodes, the line through e(0) = 0 below
+// tmin (the first segment when tmin = 0), the last segment's slope above tmax
+template <class MT> ICE_DEV double mat_e(const MT& m, double T) {
+  const double* t = prop_col(m, kPropE);
+  const int lo = m.tmin, hi = m.tmax, n = hi - lo;
+  if (T != T) return T;
+  if (T < (double)lo) return lo > 0 ? t[0] * (T / (double)lo) : (t[1] - t[0]) * T;
+  if (T >= (double)hi) return t[n] + (t[n] - t[n - 1]) * (T - (double)hi);
+  int i = (int)T;
+  return t[i - lo] + (t[i + 1 - lo] - t[i - lo]) * (T - (double)i);
+}
+
+// mat_T_from_e: the inverse of mat_e, the bracketing node by bisection
+template <class MT> ICE_DEV double mat_T_from_e(const MT& m, double e) {
+  const double* t = prop_col(m, kPropE);
+  const int lo = m.tmin, hi = m.tmax, n = hi - lo;
+  if (e != e) return e;
+  if (e < t[0]) return lo > 0 ? (double)lo * (e / t[0]) : e / (t[1] - t[0]);
+  if (e >= t[n]) return (double)hi + (e - t[n]) / (t[n] - t[n - 1]);
+  int i = lo, j = hi;
+  while (j - i > 1) {
+    int k = (i + j) / 2;
+    if (t[k - lo] <= e) i = k;
+    else j = k;
+  }
+  return (double)i + (e - t[i - lo]) / (t[i + 1 - lo] - t[i - lo]);
+}
+
+// The energy of the closures per unit mass, cs T or e(T) (cs_varies)
+template <class MT> ICE_DEV double energy_of(const MT& m, double T) { return (m.flags & kCsVaries) ? mat_e(m, T) : m.cs * T; }
+// a rho cs T / rho e(T) product in the reference's operand order
+template <class MT> ICE_DEV double energy_times(const MT& m, double a, double T) {
+  return (m.flags & kCsVaries) ? a * mat_e(m, T) : a * m.cs * T;
+}
+// T from the internal energy per unit volume: mat_T_from_rhoe (cs_varies) or rhoe / (cs rho + eps)
+template <class MT> ICE_DEV double temperature_of(const MT& m, double rhoe, double rho) {
+  if (m.flags & kCsVaries) {
+    const double* t = prop_col(m, kPropE);
+    return mat_T_from_e(m, rhoe / (rho + kEps / (t[1] - t[0])));
+  }
+  return rhoe / (m.cs * rho + kEps);
+}
+template <class MT> ICE_DEV double mat_rho(const MT& m, double T) { return (m.flags & kRhoVaries) ? prop_value(m, kPropRho, T) : m.rho; }
+template <class MT> ICE_DEV double mat_lv(const MT& m, double T) { return (m.flags & kUseLv) ? prop_value(m, kPropLv, T) : m.lv; }
+template <class MT> ICE_DEV double mat_emiss(const MT& m, double T) {
+  return (m.flags & kUseEmiss) ? prop_value(m, kPropEmiss, T) : m.emiss;
+}
+
 // ------------------------------------------------------------- closures
-template <int M> ICE_DEV void prim2cons(const double* q, double* c, const MatDev& m) {
+template <int M, class MT> ICE_DEV void prim2cons(const double* q, double* c, const MT& m) {
   double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
   c[0] = q[0];
   c[1] = q[0] * q[1];
   c[2] = q[0] * q[2];
   c[3] = q[0] * q[3];
   if constexpr (M == 0) {
-    c[4] = q[0] * m.cs * q[4] + 0.5 * q[0] * n2;
+    c[4] = energy_times(m, q[0], q[4]) + 0.5 * q[0] * n2;
     c[5] = q[5];
   } else if constexpr (M == 1) {
     c[4] = q[0] * n2 + 3.0 * q[4];
-    c[5] = q[0] * m.cs * q[5] + 0.5 * c[4];
+    c[5] = energy_times(m, q[0], q[5]) + 0.5 * c[4];
     c[6] = q[6];
   } else {
     c[4] = q[0] * q[1] * q[1] + q[4];
@@ -95,12 +171,12 @@ template <int M> ICE_DEV void prim2cons(const double* q, double* c, const MatDev
     c[7] = q[0] * q[2] * q[2] + q[7];
     c[8] = q[0] * q[2] * q[3] + q[8];
     c[9] = q[0] * q[3] * q[3] + q[9];
-    c[10] = q[0] * m.cs * q[10] + 0.5 * (c[4] + c[7] + c[9]);
+    c[10] = energy_times(m, q[0], q[10]) + 0.5 * (c[4] + c[7] + c[9]);
     c[11] = q[11];
   }
 }
 
-template <int M> ICE_DEV void cons2prim(const double* c, double* q, const MatDev& m) {
+template <int M, class MT> ICE_DEV void cons2prim(const double* c, double* q, const MT& m) {
   constexpr int NC = Closure<M>::NC;
   q[0] = c[0];
   q[1] = c[1] / (q[0] + kEps);
@@ -108,7 +184,7 @@ template <int M> ICE_DEV void cons2prim(const double* c, double* q, const MatDev
   q[3] = c[3] / (q[0] + kEps);
   if constexpr (M == 0) {
     double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-    q[4] = (c[4] - 0.5 * q[0] * n2) / (m.cs * q[0] + kEps);
+    q[4] = temperature_of(m, c[4] - 0.5 * q[0] * n2, q[0]);
     q[5] = c[5];
     if (q[0] < kEps) {
 #pragma unroll
@@ -120,7 +196,7 @@ template <int M> ICE_DEV void cons2prim(const double* c, double* q, const MatDev
     double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
     q[4] = 1.0 / 3.0 * (c[4] - q[0] * n2);
     if (q[0] <= kRhoEmpty && q[4] > q[0] * kThetaCap) q[4] = q[0] * kThetaCap;
-    q[5] = (c[5] - 0.5 * c[4]) / (m.cs * q[0] + kEps);
+    q[5] = temperature_of(m, c[5] - 0.5 * c[4], q[0]);
     q[6] = c[6];
     if (q[0] < kEps) {
 #pragma unroll
@@ -136,7 +212,7 @@ template <int M> ICE_DEV void cons2prim(const double* c, double* q, const MatDev
     q[7] = c[7] - q[0] * q[2] * q[2];
     q[8] = c[8] - q[0] * q[2] * q[3];
     q[9] = c[9] - q[0] * q[3] * q[3];
-    q[10] = (c[10] - 0.5 * (c[4] + c[7] + c[9])) / (m.cs * q[0] + kEps);
+    q[10] = temperature_of(m, c[10] - 0.5 * (c[4] + c[7] + c[9]), q[0]);
     q[11] = c[11];
     if (q[0] <= kRhoEmpty && q[4] + q[7] + q[9] > 3.0 * q[0] * kThetaCap) {
       double f = 3.0 * q[0] * kThetaCap / (q[4] + q[7] + q[9]);
@@ -185,7 +261,7 @@ template <int M> ICE_DEV double wavespeed(const double* q, const double* n) {
   }
 }
 
-template <int M> ICE_DEV void physflux(const double* q, const double* n, double* f, const MatDev& m) {
+template <int M, class MT> ICE_DEV void physflux(const double* q, const double* n, double* f, const MT& m) {
   double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
   double un = q[1] * n[0] + q[2] * n[1] + q[3] * n[2];
   f[0] = q[0] * un;
@@ -193,14 +269,14 @@ template <int M> ICE_DEV void physflux(const double* q, const double* n, double*
     f[1] = f[0] * q[1];
     f[2] = f[0] * q[2];
     f[3] = f[0] * q[3];
-    f[4] = f[0] * (0.5 * n2 + m.cs * q[4]);
+    f[4] = f[0] * (0.5 * n2 + energy_of(m, q[4]));
     f[5] = q[5] * un;
   } else if constexpr (M == 1) {
     f[1] = f[0] * q[1] + q[4] * n[0];
     f[2] = f[0] * q[2] + q[4] * n[1];
     f[3] = f[0] * q[3] + q[4] * n[2];
     f[4] = f[0] * n2 + 5.0 * q[4] * un;
-    f[5] = f[0] * (0.5 * n2 + m.cs * q[5]) + 2.5 * q[4] * un;
+    f[5] = f[0] * (0.5 * n2 + energy_of(m, q[5])) + 2.5 * q[4] * un;
     f[6] = q[6] * un;
   } else {
     double p1n = q[4] * n[0] + q[5] * n[1] + q[6] * n[2];
@@ -215,14 +291,14 @@ template <int M> ICE_DEV void physflux(const double* q, const double* n, double*
     f[7] = f[0] * q[2] * q[2] + un * q[7] + q[2] * p2n + q[2] * p2n;
     f[8] = f[0] * q[2] * q[3] + un * q[8] + q[2] * p3n + q[3] * p2n;
     f[9] = f[0] * q[3] * q[3] + un * q[9] + q[3] * p3n + q[3] * p3n;
-    f[10] = f[0] * (0.5 * n2 + 0.5 * (q[4] + q[7] + q[9]) / (q[0] + kEps) + m.cs * q[10]) + q[1] * p1n +
+    f[10] = f[0] * (0.5 * n2 + 0.5 * (q[4] + q[7] + q[9]) / (q[0] + kEps) + energy_of(m, q[10])) + q[1] * p1n +
             q[2] * p2n + q[3] * p3n;
     f[11] = q[11] * un;
   }
 }
 
 // source_make of the closure; F = force(1:6) of compute_source_
-template <int M> ICE_DEV void sourcemake(const double* q, const double* F, double* s, const MatDev& m) {
+template <int M, class MT> ICE_DEV void sourcemake(const double* q, const double* F, double* s, const MT& m) {
   double it = F[5] + kEps;
   s[0] = -F[0];
   s[1] = -F[0] * q[1] + q[0] * F[1] / it;
@@ -230,13 +306,13 @@ template <int M> ICE_DEV void sourcemake(const double* q, const double* F, doubl
   s[3] = -F[0] * q[3] + q[0] * F[3] / it;
   if constexpr (M == 0) {
     double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-    s[4] = -(F[0] * (0.5 * n2 + m.cs * q[4])) - F[0] * m.lv + F[4] + q[0] * (F[1] * q[1] + F[2] * q[2] + F[3] * q[3]) / it;
+    s[4] = -(F[0] * (0.5 * n2 + energy_of(m, q[4]))) - F[0] * mat_lv(m, q[4]) + F[4] + q[0] * (F[1] * q[1] + F[2] * q[2] + F[3] * q[3]) / it;
     s[5] = 0.0;
   } else if constexpr (M == 1) {
     double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
     s[4] = -(F[0] * (n2 + 3.0 * q[4] / q[0])) + 2.0 * q[0] * (F[1] * q[1] + F[2] * q[2] + F[3] * q[3]) / it -
            6.0 * q[4] / it;
-    s[5] = -(F[0] * m.cs * q[5]) - F[0] * m.lv + F[4] + 0.5 * s[4];
+    s[5] = -energy_times(m, F[0], q[5]) - F[0] * mat_lv(m, q[5]) + F[4] + 0.5 * s[4];
     s[6] = 0.0;
   } else {
     s[4] = -(F[0] * (q[1] * q[1] + q[4] / q[0])) + q[0] * (F[1] * q[1] + F[1] * q[1]) / it - 2.0 * q[4] / it;
@@ -245,7 +321,7 @@ template <int M> ICE_DEV void sourcemake(const double* q, const double* F, doubl
     s[7] = -(F[0] * (q[2] * q[2] + q[7] / q[0])) + q[0] * (F[2] * q[2] + F[2] * q[2]) / it - 2.0 * q[7] / it;
     s[8] = -(F[0] * (q[2] * q[3] + q[8] / q[0])) + q[0] * (F[3] * q[2] + F[2] * q[3]) / it - 2.0 * q[8] / it;
     s[9] = -(F[0] * (q[3] * q[3] + q[9] / q[0])) + q[0] * (F[3] * q[3] + F[3] * q[3]) / it - 2.0 * q[9] / it;
-    s[10] = -(F[0] * m.cs * q[10]) - F[0] * m.lv + F[4] + 0.5 * s[4] + 0.5 * s[7] + 0.5 * s[9];
+    s[10] = -energy_times(m, F[0], q[10]) - F[0] * mat_lv(m, q[10]) + F[4] + 0.5 * s[4] + 0.5 * s[7] + 0.5 * s[9];
     s[11] = 0.0;
   }
 }
@@ -266,8 +342,8 @@ ICE_DEV void mirror6(const double* p, const double* n, double* pm) {
 
 // ------------------------------------------------------------- Riemann
 // S: 0 Saurel (MK only), 1 Rusanov, 2 HLLE
-template <int M, int S> ICE_DEV void riemann(const double* q1, const double* q4, const double* n, double* f,
-                                             const MatDev& m) {
+template <int M, int S, class MT>
+ICE_DEV void riemann(const double* q1, const double* q4, const double* n, double* f, const MT& m) {
   constexpr int NC = Closure<M>::NC;
   double vn1 = q1[1] * n[0] + q1[2] * n[1] + q1[3] * n[2];
   double vn4 = q4[1] * n[0] + q4[2] * n[1] + q4[3] * n[2];
@@ -283,7 +359,7 @@ template <int M, int S> ICE_DEV void riemann(const double* q1, const double* q4,
       f[1] = uu * f[0];
       f[2] = vv * f[0];
       f[3] = ww * f[0];
-      f[4] = f[0] * (0.5 * (uu * uu + vv * vv + ww * ww) + m.cs * qu[4]);
+      f[4] = f[0] * (0.5 * (uu * uu + vv * vv + ww * ww) + energy_of(m, qu[4]));
       f[5] = qu[5] * vn;
     } else {
 #pragma unroll
@@ -519,20 +595,20 @@ namespace ice {
 // Component v of the conservative vector and of the physical flux, evaluated
 // with exactly the operations of prim2cons / physflux, so a Riemann solver can
 // emit one flux component at a time without holding the four vectors.
-template <int M> ICE_DEV double cons_v(int v, const double* q, const MatDev& m) {
+template <int M, class MT> ICE_DEV double cons_v(int v, const double* q, const MT& m) {
   if (v == 0) return q[0];
   if (v <= 3) return q[0] * q[v];
   if constexpr (M == 0) {
     if (v == 4) {
       double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-      return q[0] * m.cs * q[4] + 0.5 * q[0] * n2;
+      return energy_times(m, q[0], q[4]) + 0.5 * q[0] * n2;
     }
     return q[5];
   } else if constexpr (M == 1) {
     double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
     double c4 = q[0] * n2 + 3.0 * q[4];
     if (v == 4) return c4;
-    if (v == 5) return q[0] * m.cs * q[5] + 0.5 * c4;
+    if (v == 5) return energy_times(m, q[0], q[5]) + 0.5 * c4;
     return q[6];
   } else {
     switch (v) {
@@ -544,7 +620,7 @@ template <int M> ICE_DEV double cons_v(int v, const double* q, const MatDev& m) 
       case 9: return q[0] * q[3] * q[3] + q[9];
       case 10: {
         double c4 = q[0] * q[1] * q[1] + q[4], c7 = q[0] * q[2] * q[2] + q[7], c9 = q[0] * q[3] * q[3] + q[9];
-        return q[0] * m.cs * q[10] + 0.5 * (c4 + c7 + c9);
+        return energy_times(m, q[0], q[10]) + 0.5 * (c4 + c7 + c9);
       }
       default: return q[11];
     }
@@ -569,17 +645,17 @@ template <int M> ICE_DEV FluxScal flux_scal(const double* q, const double* n) {
   }
   return s;
 }
-template <int M>
-ICE_DEV double flux_v(int v, const double* q, const double* n, const FluxScal& s, const MatDev& m) {
+template <int M, class MT>
+ICE_DEV double flux_v(int v, const double* q, const double* n, const FluxScal& s, const MT& m) {
   if (v == 0) return s.f0;
   if constexpr (M == 0) {
     if (v <= 3) return s.f0 * q[v];
-    if (v == 4) return s.f0 * (0.5 * s.n2 + m.cs * q[4]);
+    if (v == 4) return s.f0 * (0.5 * s.n2 + energy_of(m, q[4]));
     return q[5] * s.un;
   } else if constexpr (M == 1) {
     if (v <= 3) return s.f0 * q[v] + q[4] * n[v - 1];
     if (v == 4) return s.f0 * s.n2 + 5.0 * q[4] * s.un;
-    if (v == 5) return s.f0 * (0.5 * s.n2 + m.cs * q[5]) + 2.5 * q[4] * s.un;
+    if (v == 5) return s.f0 * (0.5 * s.n2 + energy_of(m, q[5])) + 2.5 * q[4] * s.un;
     return q[6] * s.un;
   } else {
     switch (v) {
@@ -593,7 +669,7 @@ ICE_DEV double flux_v(int v, const double* q, const double* n, const FluxScal& s
       case 8: return s.f0 * q[2] * q[3] + s.un * q[8] + q[2] * s.p3n + q[3] * s.p2n;
       case 9: return s.f0 * q[3] * q[3] + s.un * q[9] + q[3] * s.p3n + q[3] * s.p3n;
       case 10:
-        return s.f0 * (0.5 * s.n2 + 0.5 * (q[4] + q[7] + q[9]) / (q[0] + kEps) + m.cs * q[10]) + q[1] * s.p1n +
+        return s.f0 * (0.5 * s.n2 + 0.5 * (q[4] + q[7] + q[9]) / (q[0] + kEps) + energy_of(m, q[10])) + q[1] * s.p1n +
                q[2] * s.p2n + q[3] * s.p3n;
       default: return q[11] * s.un;
     }
@@ -602,9 +678,8 @@ ICE_DEV double flux_v(int v, const double* q, const double* n, const FluxScal& s
 
 // The Riemann flux of states ql | qr times the face area, each component added
 // to (MODE 0) or subtracted from (MODE 1) the accumulator r, or stored in r (MODE 2).
-template <int M, int S, int MODE>
-ICE_DEV void riemann_acc(const double* q1, const double* q4, const double* n, double area, double* r,
-                         const MatDev& m) {
+template <int M, int S, int MODE, class MT>
+ICE_DEV void riemann_acc(const double* q1, const double* q4, const double* n, double area, double* r, const MT& m) {
   constexpr int NC = Closure<M>::NC;
   if constexpr (S == 0) {
     double f[NC];

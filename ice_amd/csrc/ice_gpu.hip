@@ -16,6 +16,7 @@
 #include "batched.hpp"
 #include "inflow400.hpp"
 #include "launch.hpp"
+#include "multigrid.hpp"
 
 using namespace ice;
 
@@ -32,6 +33,8 @@ struct FamDev {
   long long* bcell_g = nullptr;
   int ngrp = 0, nown = 0, nbcell = 0, nconn = 0;
   int nc = 0;
+  double* prop = nullptr;   // the material's property table (kPropCols columns), or null
+  int prop_tmin = 0, prop_tmax = 0, prop_flags = 0;
 };
 
 // The 400 records of a family and its boundaries' accumulators and totals
@@ -39,6 +42,8 @@ struct FamDev {
 struct I400Fam {
   int* rec = nullptr;
   int n = 0;
+  I400Chunk* chunks = nullptr;
+  int nchunk = 0;
   I400Acc* acc = nullptr;
   I400Tot* tot = nullptr;
 };
@@ -95,6 +100,10 @@ struct ice_gpu_ctx {
   int retrain_every = 0, retrain_epochs = 20;  // online retraining of the trained networks
   int tile[3] = {-1, -1, -1};  // cells per stage tile; -1 default shape, 0 per-cell kernel
   int dispatch = 0;            // batched stage: 0 persistent queues, 1 one workgroup per item
+  double res_threshold = 0.0;  // res-threshold of a steady run (0: never stop on the residual)
+  double res_reference = 0.0;  // the first non-zero density residual of the run
+  int res_stopped = 0;         // the last ice_gpu_step ended on the residual
+  double *res_part = nullptr, *res_sums = nullptr;  // residual-norm partial sums
   std::vector<I400Fam> i400;   // per family
   ice_inflow400_line_t* i400_lines = nullptr;  // the line log, ICE_INFLOW400_LOG entries
   int* i400_n = nullptr;                        // lines logged since the last read
@@ -294,15 +303,31 @@ int build_lists(ice_gpu_ctx* c) {
     for (size_t m = 0; m < bg.size(); ++m) bidx[bg[m]] = (int)m;
     HIPCHK(hipMemcpy(F.bidx, bidx.data(), c->N * sizeof(int), hipMemcpyHostToDevice));
     // the family's 400 records in table order (list_records, Mod_Inflow400.f90:185-200)
+    // (sorted by boundary, block then face, and cut into chunks: the counts and
+    // maxima of a boundary do not depend on the order of its records)
     std::vector<int> r400;
     for (int n0 : grp)
       if (c->rec[n0].v[ICE_R_TYPE] == 400) r400.push_back(n0);
+    auto bnd_of = [&](int n0) { return (c->rec[n0].v[ICE_R_B] - 1) * 6 + (c->rec[n0].v[ICE_R_F] - 1); };
+    std::stable_sort(r400.begin(), r400.end(), [&](int x, int y) { return bnd_of(x) < bnd_of(y); });
+    std::vector<I400Chunk> chunks;
+    for (size_t s0 = 0; s0 < r400.size();) {
+      size_t e0 = s0;
+      while (e0 < r400.size() && bnd_of(r400[e0]) == bnd_of(r400[s0])) ++e0;
+      for (size_t a = s0; a < e0; a += kI400Chunk)
+        chunks.push_back(I400Chunk{bnd_of(r400[s0]), (int)a, (int)std::min(e0, a + kI400Chunk), 0});
+      s0 = e0;
+    }
     I400Fam& I = c->i400[p - 1];
     I.n = (int)r400.size();
+    I.nchunk = (int)chunks.size();
     if (I.n) {
       size_t nbnd = 6 * c->blk.size();
-      if (dalloc(c, &I.rec, r400.size()) || dalloc(c, &I.acc, nbnd) || dalloc(c, &I.tot, nbnd)) return 1;
+      if (dalloc(c, &I.rec, r400.size()) || dalloc(c, &I.acc, nbnd) || dalloc(c, &I.tot, nbnd) ||
+          dalloc(c, &I.chunks, chunks.size()))
+        return 1;
       HIPCHK(hipMemcpy(I.rec, r400.data(), r400.size() * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(I.chunks, chunks.data(), chunks.size() * sizeof(I400Chunk), hipMemcpyHostToDevice));
       HIPCHK(hipMemset(I.acc, 0, nbnd * sizeof(I400Acc)));
       HIPCHK(hipMemset(I.tot, 0, nbnd * sizeof(I400Tot)));
     }
@@ -320,7 +345,7 @@ int build_lists(ice_gpu_ctx* c) {
 void launch_i400(ice_gpu_ctx* c, int p, const FamArgs& a) {
   const I400Fam& I = c->i400[p];
   if (!I.n) return;
-  hipLaunchKernelGGL(k_i400_count, dim3(grid_of(I.n)), dim3(kThreads), 0, c->stream, a, c->fams[p].closure, I.rec, I.n,
+  hipLaunchKernelGGL(k_i400_count, dim3(I.nchunk), dim3(256), 0, c->stream, a, c->fams[p].closure, I.rec, I.chunks,
                      I.acc, I.tot);
   hipLaunchKernelGGL(k_i400_fold, dim3(1), dim3(256), 0, c->stream, (int)(6 * c->blk.size()), p + 1, I.acc, I.tot,
                      c->i400_lines, c->i400_n, c->d_st);
@@ -343,6 +368,7 @@ FamArgs fam_args(ice_gpu_ctx* c, int p) {
   a.bfirst = F.bfirst; a.brec = F.brec; a.bcell_g = F.bcell_g; a.nbcell = F.nbcell;
   a.st = c->d_st;
   a.mat.rho = fd.rho_mat; a.mat.cs = fd.cs; a.mat.lv = fd.lv; a.mat.emiss = fd.emiss;
+  a.mat.tab = F.prop; a.mat.tmin = F.prop_tmin; a.mat.tmax = F.prop_tmax; a.mat.flags = F.prop_flags;
   for (int m = 0; m < 8; ++m) a.mat.ep[m] = fd.ep[m];
   a.mat.evap = fd.evap_select; a.mat.intf = fd.intf_select; a.mat.surrogate = fd.surrogate;
   a.limiter = c->cfg.limiter; a.nrk = c->cfg.nrk; a.time_accurate = c->cfg.time_accurate;
@@ -459,10 +485,22 @@ Tile stage_tile(ice_gpu_ctx* c, const BlkDev& B) {
   return T;
 }
 
+// Families with tabulated properties (flags set) run the per-cell stage kernel,
+// which takes the table's branches; the tiled and batched kernels are compiled
+// for constant properties only.
+bool any_table(const ice_gpu_ctx* c) {
+  for (const FamDev& F : c->fam)
+    if (F.prop_flags) return true;
+  return false;
+}
+
 void dispatch_stage(ice_gpu_ctx* c, FamArgs& a) {
   const FamDev& F = c->fam[a.fam];
   std::vector<Tile> tiles(c->blk.size());
-  for (size_t b = 0; b < c->blk.size(); ++b) tiles[b] = stage_tile(c, c->blk[b]);
+  for (size_t b = 0; b < c->blk.size(); ++b) {
+    tiles[b] = stage_tile(c, c->blk[b]);
+    if (F.prop_flags) tiles[b].tx = 0;
+  }
   StageLaunch L;
   L.stream = c->stream;
   L.blk = c->blk.data();
@@ -519,6 +557,9 @@ int plain_step(ice_gpu_ctx* c) {
 
 // online retraining of every trained network on the current state (surrogate.hpp)
 int retrain_networks(ice_gpu_ctx* c);
+namespace {
+int residual_norm(ice_gpu_ctx* c, double* total, double* fam);
+}
 
 // ================================================================= C-ABI
 extern "C" {
@@ -830,7 +871,7 @@ int ice_gpu_put_tau(ice_gpu_ctx* c, int b, int p, const double* tau) {
 }
 
 int ice_gpu_set_keep_source(ice_gpu_ctx* c, int on) {
-  c->keep_source = on ? 1 : 0;
+  c->keep_source = on == 2 ? 2 : (on ? 1 : 0);
   for (auto& F : c->fam)
     if (on && !F.src) {
       if (dalloc(c, &F.src, (size_t)F.nc * c->N) || fill(c, F.src, (long long)F.nc * c->N, 0.0)) return 1;
@@ -845,10 +886,23 @@ int ice_gpu_step(ice_gpu_ctx* c, int nsteps, ice_status_t* st) {
   for (FamArgs* d : c->kev_args) (void)hipFree(d);
   c->kev_args.clear();
   HIPCHK(hipEventRecord(c->ev0, c->stream));
+  c->res_stopped = 0;
+  const bool res_stop = c->res_threshold > 0.0 && !c->cfg.time_accurate;
   for (int s = 0; s < nsteps; ++s) {
-    int rc = (c->cfg.mode == ICE_MODE_BATCHED) ? batched_step(c) : plain_step(c);
+    int rc = (c->cfg.mode == ICE_MODE_BATCHED && !any_table(c)) ? batched_step(c) : plain_step(c);
     if (rc) return rc;
     if (c->retrain_every > 0 && c->iter % c->retrain_every == 0 && retrain_networks(c)) return 1;
+    if (res_stop) {
+      // Mod_Explicit.f90:196-225: a steady run ends when its density residual has
+      // fallen by res-threshold from the first non-zero one
+      std::vector<double> tot(5), fam(5 * c->cfg.ngroups);
+      if (residual_norm(c, tot.data(), fam.data())) return 1;
+      if (c->res_reference == 0.0) c->res_reference = tot[0];
+      if (c->res_reference > 0.0 && tot[0] <= c->res_threshold * c->res_reference) {
+        c->res_stopped = 1;
+        break;
+      }
+    }
   }
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -909,12 +963,22 @@ int ice_gpu_get_source(ice_gpu_ctx* c, int b, int p, double* source, double* tau
 int ice_gpu_residual_norm(ice_gpu_ctx* c, double* total, double* fam) {
   if (!c) return 1;
   if (!total || !fam) return fail(c, "[ICE::residual_norm] null output");
+  return residual_norm(c, total, fam);
+}
+
+}  // extern "C"
+
+namespace {
+
+// residual_norm (Mod_Diagnostic.f90:22-69) of the last step into total[5] and
+// fam[5 * ngroups], with the context's partial-sum buffers
+int residual_norm(ice_gpu_ctx* c, double* total, double* fam) {
   const int nb = (int)c->blk.size(), ng = c->cfg.ngroups;
-  double *part = nullptr, *sums = nullptr;
-  if (dalloc(c, &part, (size_t)nb * ng * kResWG * 5) || dalloc(c, &sums, (size_t)nb * ng * 5)) {
-    if (part) (void)hipFree(part);
-    return 1;
+  if (!c->res_part) {
+    if (dalloc(c, &c->res_part, (size_t)nb * ng * kResWG * 5) || dalloc(c, &c->res_sums, (size_t)nb * ng * 5))
+      return 1;
   }
+  double *part = c->res_part, *sums = c->res_sums;
   for (int p = 0; p < ng; ++p) {
     const FamDev& F = c->fam[p];
     hipLaunchKernelGGL(k_resnorm_part, dim3(kResWG, nb), dim3(256), 0, c->stream, c->d_blk, F.buf[F.cur],
@@ -924,8 +988,6 @@ int ice_gpu_residual_norm(ice_gpu_ctx* c, double* total, double* fam) {
   std::vector<double> h((size_t)nb * ng * 5);
   hipError_t e = hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(part);
-  (void)hipFree(sums);
   if (e != hipSuccess) return fail(c, "[ICE::residual_norm] %s", hipGetErrorString(e));
   // totals over blocks then families, family sums over blocks (Mod_Diagnostic.f90:106-127)
   for (int m = 0; m < 5; ++m) total[m] = 0.0;
@@ -939,6 +1001,26 @@ int ice_gpu_residual_norm(ice_gpu_ctx* c, double* total, double* fam) {
       }
   for (int m = 0; m < 5; ++m) total[m] = std::sqrt(total[m]);
   for (int m = 0; m < 5 * ng; ++m) fam[m] = std::sqrt(fam[m]);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ice_gpu_set_res_threshold(ice_gpu_ctx* c, double threshold) {
+  if (!c) return 1;
+  if (!(threshold >= 0.0)) return fail(c, "[ICE::set_res_threshold] res-threshold must be >= 0");
+  c->res_threshold = threshold;
+  c->res_reference = 0.0;
+  c->res_stopped = 0;
+  return 0;
+}
+
+int ice_gpu_res_stop(ice_gpu_ctx* c, int32_t* stopped, double* reference) {
+  if (!c) return 1;
+  if (stopped) *stopped = c->res_stopped;
+  if (reference) *reference = c->res_reference;
   return 0;
 }
 
@@ -1132,6 +1214,93 @@ int ice_gpu_set_kernel_timing(ice_gpu_ctx* c, int on) {
   return 0;
 }
 
+// Two contexts one level apart (Check_Multigrid, Lib_Multigrid.f90:12-35): same
+// device, families and blocks, each coarse block half the fine one per direction
+int level_pair(ice_gpu_ctx* f, ice_gpu_ctx* c) {
+  if (!f || !c) return 1;
+  if (f->device != c->device) return fail(f, "[ICE::Multigrid] the levels are on different devices");
+  if (f->cfg.ngroups != c->cfg.ngroups || f->blk.size() != c->blk.size())
+    return fail(f, "[ICE::Multigrid] the levels have different families or blocks");
+  for (int p = 0; p < f->cfg.ngroups; ++p)
+    if (f->fams[p].closure != c->fams[p].closure) return fail(f, "[ICE::Multigrid] family %d changes closure", p + 1);
+  for (size_t b = 0; b < f->blk.size(); ++b) {
+    const BlkDev &F = f->blk[b], &C = c->blk[b];
+    bool ok = F.ni % 2 == 0 && F.nj % 2 == 0 && C.ni * 2 == F.ni && C.nj * 2 == F.nj &&
+              (F.nk == 1 ? C.nk == 1 : (F.nk % 2 == 0 && C.nk * 2 == F.nk));
+    if (!ok) return fail(f, " [ERROR] [ICE::Check_Multigrid] block %d not divisible by 2^(MGL-1)", (int)b + 1);
+  }
+  HIPCHK(hipStreamSynchronize(f->stream));
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(f, "[ICE::Multigrid] %s", hipGetErrorString(e));
+  return 0;
+}
+
+int ice_gpu_restrict(ice_gpu_ctx* fine, ice_gpu_ctx* coarse) {
+  if (level_pair(fine, coarse)) return 1;
+  ice_gpu_ctx* c = fine;
+  for (int p = 0; p < fine->cfg.ngroups; ++p) {
+    FamArgs fa = fam_args(fine, p), ca = fam_args(coarse, p);
+    for (size_t b = 0; b < fine->blk.size(); ++b) {
+      dim3 grid(grid_of(coarse->blk[b].cells)), blk(kThreads);
+      int cl = fine->fams[p].closure;
+      if (cl == ICE_MK) hipLaunchKernelGGL(k_restrict<0>, grid, blk, 0, fine->stream, fa, ca, (int)b);
+      else if (cl == ICE_IG) hipLaunchKernelGGL(k_restrict<1>, grid, blk, 0, fine->stream, fa, ca, (int)b);
+      else hipLaunchKernelGGL(k_restrict<2>, grid, blk, 0, fine->stream, fa, ca, (int)b);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(fine->stream));
+  return 0;
+}
+
+int ice_gpu_prolong(ice_gpu_ctx* coarse, ice_gpu_ctx* fine) {
+  if (level_pair(fine, coarse)) return 1;
+  ice_gpu_ctx* c = fine;
+  for (int p = 0; p < fine->cfg.ngroups; ++p) {
+    FamArgs fa = fam_args(fine, p), ca = fam_args(coarse, p);
+    for (size_t b = 0; b < fine->blk.size(); ++b) {
+      dim3 grid(grid_of(coarse->blk[b].cells)), blk(kThreads);
+      int nc = nc_of(fine->fams[p].closure);
+      if (nc == 6) hipLaunchKernelGGL(k_prolong<6>, grid, blk, 0, fine->stream, ca, fa, (int)b);
+      else if (nc == 7) hipLaunchKernelGGL(k_prolong<7>, grid, blk, 0, fine->stream, ca, fa, (int)b);
+      else hipLaunchKernelGGL(k_prolong<12>, grid, blk, 0, fine->stream, ca, fa, (int)b);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(fine->stream));
+  return 0;
+}
+
+int ice_gpu_set_properties(ice_gpu_ctx* c, int p, int t_min, int n, const double* cols, int32_t flags) {
+  if (!c) return 1;
+  if (p < 1 || p > c->cfg.ngroups) return fail(c, "[ICE::set_properties] family %d out of range", p);
+  if (n < 2) return fail(c, "[ICE::Load_Table] a property table needs at least 2 rows (found %d)", n);
+  if (t_min < 0) return fail(c, "[ICE::Load_Table] negative temperatures in the property table");
+  if (flags & ~(ICE_PROP_RHO_VARIES | ICE_PROP_CS_VARIES | ICE_PROP_USE_PSAT | ICE_PROP_USE_LV | ICE_PROP_USE_CPV |
+                ICE_PROP_USE_EMISS))
+    return fail(c, "[ICE::set_properties] unknown flags %d", (int)flags);
+  static_assert(ICE_PROP_NCOL == kPropCols, "column count");
+  static_assert(ICE_PROP_RHO_VARIES == kRhoVaries && ICE_PROP_CS_VARIES == kCsVaries && ICE_PROP_USE_PSAT == kUsePsat &&
+                    ICE_PROP_USE_LV == kUseLv && ICE_PROP_USE_CPV == kUseCpv && ICE_PROP_USE_EMISS == kUseEmiss,
+                "flag values");
+  FamDev& F = c->fam[p - 1];
+  if (F.prop) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(F.prop));
+    F.prop = nullptr;
+  }
+  if (dalloc(c, &F.prop, (size_t)ICE_PROP_NCOL * n)) return 1;
+  HIPCHK(hipMemcpy(F.prop, cols, (size_t)ICE_PROP_NCOL * n * sizeof(double), hipMemcpyHostToDevice));
+  F.prop_tmin = t_min;
+  F.prop_tmax = t_min + n - 1;
+  F.prop_flags = flags;
+  // mat_rho without rho_varies and mat_cp_const take the table's first row (Lib_Properties.f90:18-53)
+  c->fams[p - 1].rho_mat = cols[(size_t)ICE_PROP_RHO * n];
+  c->fams[p - 1].cs = cols[(size_t)ICE_PROP_CP * n];
+  batched_destroy(c);  // captured graphs hold the old material
+  return 0;
+}
+
 int ice_gpu_inflow400_lines(ice_gpu_ctx* c, ice_inflow400_line_t* out, int max, int32_t* n, int64_t* dropped) {
   if (!c) return 1;
   *n = 0;
@@ -1196,15 +1365,15 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
   for (auto& f : c->fam) {
     for (int m = 0; m < 3; ++m) F(f.buf[m]);
     F(f.beta); F(f.dt); F(f.tau); F(f.src); F(f.R); F(f.T); F(f.bsum); F(f.connF); F(f.inj); F(f.bidx);
-    F(f.grp); F(f.own); F(f.bfirst); F(f.brec); F(f.bcell_g);
+    F(f.grp); F(f.own); F(f.bfirst); F(f.brec); F(f.bcell_g); F(f.prop);
   }
   for (auto& S : c->surr) {
     F(S.P); F(S.m); F(S.v); F(S.part); F(S.loss); F(S.w); F(S.stats);
   }
   for (auto& I : c->i400) {
-    F(I.rec); F(I.acc); F(I.tot);
+    F(I.rec); F(I.acc); F(I.tot); F(I.chunks);
   }
-  F(c->i400_lines); F(c->i400_n);
+  F(c->i400_lines); F(c->i400_n); F(c->res_part); F(c->res_sums);
   batched_destroy(c);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -1,9 +1,10 @@
 // Particles entering through extrapolation faces (400): count_400_inflow
 // (src/lib/diagnostic/Mod_Inflow400.f90:52-128) as two small kernels per stage
 // of a family, both capturable in a graph (no memset, no host read):
-//   k_i400_count  one thread per 400 record of the family: the record's measure,
-//                 folded into its boundary's accumulators with atomics (counts,
-//                 and maxima of positive doubles as the maxima of their bits);
+//   k_i400_count  one workgroup per chunk of one boundary's 400 records: the
+//                 records' measures reduced in the workgroup, then folded into the
+//                 boundary's accumulators with four atomics (counts, and maxima of
+//                 positive doubles as the maxima of their bits);
 //   k_i400_fold   one workgroup: every boundary in (block, face) order folds its
 //                 accumulators into its totals, logs a line where the reference
 //                 prints one (slots by a workgroup prefix sum, so the log keeps
@@ -68,20 +69,65 @@ ICE_DEV double i400_measure(const FamArgs& a, int closure, const DRec& r) {
   return x;
 }
 
-__global__ __launch_bounds__(256) void k_i400_count(FamArgs a, int closure, const int* __restrict__ rec400, int n,
-                                                    I400Acc* __restrict__ acc, const I400Tot* __restrict__ tot) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const DRec r = a.rec[rec400[t]];
-  double x = i400_measure(a, closure, r);
-  if (!(x > 0.0)) return;
-  const int m = (r.b - 1) * 6 + (r.f - 1);
-  const unsigned long long xb = (unsigned long long)__double_as_longlong(x);
-  atomicAdd(&acc[m].nface, 1u);
-  atomicMax(&acc[m].xs, xb);
-  if (x > 10.0 * tot[m].line_x) {
-    atomicAdd(&acc[m].nabove, 1u);
-    atomicMax(&acc[m].xa, xb);
+// A chunk of one boundary's 400 records (the host sorts a family's records by
+// boundary and cuts each boundary's run into chunks of at most kI400Chunk)
+struct I400Chunk {
+  int m, start, end, pad;
+};
+constexpr int kI400Chunk = 2048;
+
+// one workgroup of 256 per chunk: its records' counts and maxima reduced in the
+// workgroup, then four atomics into the boundary's accumulators
+__global__ __launch_bounds__(256) void k_i400_count(FamArgs a, int closure, const int* __restrict__ rec400,
+                                                    const I400Chunk* __restrict__ chunks, I400Acc* __restrict__ acc,
+                                                    const I400Tot* __restrict__ tot) {
+  __shared__ unsigned long long red[4][4];
+  const I400Chunk C = chunks[blockIdx.x];
+  const double line = 10.0 * tot[C.m].line_x;
+  unsigned long long nface = 0, nabove = 0, xs = 0, xa = 0;
+  for (int t = C.start + (int)threadIdx.x; t < C.end; t += blockDim.x) {
+    const DRec r = a.rec[rec400[t]];
+    double x = i400_measure(a, closure, r);
+    if (!(x > 0.0)) continue;
+    const unsigned long long xb = (unsigned long long)__double_as_longlong(x);
+    nface += 1;
+    xs = xb > xs ? xb : xs;
+    if (x > line) {
+      nabove += 1;
+      xa = xb > xa ? xb : xa;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nface += __shfl_xor(nface, off);
+    nabove += __shfl_xor(nabove, off);
+    unsigned long long o = __shfl_xor(xs, off);
+    xs = o > xs ? o : xs;
+    o = __shfl_xor(xa, off);
+    xa = o > xa ? o : xa;
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = nface;
+    red[1][w] = nabove;
+    red[2][w] = xs;
+    red[3][w] = xa;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      nface += red[0][k];
+      nabove += red[1][k];
+      xs = red[2][k] > xs ? red[2][k] : xs;
+      xa = red[3][k] > xa ? red[3][k] : xa;
+    }
+    if (nface) {
+      atomicAdd(&acc[C.m].nface, (unsigned)nface);
+      atomicMax(&acc[C.m].xs, xs);
+    }
+    if (nabove) {
+      atomicAdd(&acc[C.m].nabove, (unsigned)nabove);
+      atomicMax(&acc[C.m].xa, xa);
+    }
   }
 }
 

@@ -92,6 +92,15 @@ struct FamArgs {
   unsigned long long* lk_stats;  // [4] or nullptr
 };
 
+// The material of family a as a launch sees it: with tabulated properties
+// possible (TAB), or as constant properties (MatConst), for the launches that
+// serve only families without a table.
+template <bool TAB> struct MatOf { using type = MatDev; };
+template <> struct MatOf<false> { using type = MatConst; };
+template <bool TAB> ICE_DEV const typename MatOf<TAB>::type& mat_of(const FamArgs& a) {
+  return static_cast<const typename MatOf<TAB>::type&>(a.mat);
+}
+
 ICE_DEV long long gidx(const BlkDev& B, int i, int j, int k) {
   return B.off + (long long)(i + 1) + (long long)B.NI * ((long long)(j + 1) + (long long)B.NJ * (long long)(k + 1));
 }
@@ -109,7 +118,7 @@ template <int NC> ICE_DEV void store_cell(double* p, long long N, long long g, c
 // Face between padded cell g and g + stride (direction d), its lower cell at
 // index m (1-based) of n along d (Mod_Fluxes.f90:599-685); its flux times the
 // area is added to (MODE 0), subtracted from (MODE 1) or stored in (MODE 2) r.
-template <int M, int S, bool SD, int MODE, int LIM = -1>
+template <int M, int S, bool SD, int MODE, int LIM = -1, bool TAB = true>
 ICE_DEV bool face_acc(const FamArgs& a, long long g, long long stride, int d, int m, int n, double* r) {
   constexpr int NC = Closure<M>::NC;
   const long long N = a.N;
@@ -128,7 +137,7 @@ ICE_DEV bool face_acc(const FamArgs& a, long long g, long long stride, int d, in
   double dlr = 0.5 * len2;
   double ql[NC], qr[NC];
   bool ok = reconstruct_mem<M, LIM>(a.limiter, a.cur + (g - stride), stride, N, dl0, dl1, dl2, dll, dlr, bf, ql, qr);
-  riemann_acc<M, S, MODE>(ql, qr, nrm, area, r, a.mat);
+  riemann_acc<M, S, MODE>(ql, qr, nrm, area, r, mat_of<TAB>(a));
   return ok;
 }
 
@@ -138,17 +147,29 @@ ICE_DEV bool face_acc(const FamArgs& a, long long g, long long stride, int d, in
 // blowing models (Lib_Evaporation.f90), no combustion, no P2T/boiling.  With a
 // per-cell far-field vapour fraction its value replaces ep(iYinf).
 // The drop radius, slip speed and slip Reynolds number of compute_source_
-template <int M>
+template <int M, bool TAB = true>
 ICE_DEV void drop_state(const FamArgs& a, long long g, const double* q, const double* gp, double& Rp, double& slip,
                         double& Re) {
   constexpr int NC = Closure<M>::NC;
-  Rp = pow(0.75 * q[0] / (q[NC - 1] * kPi * a.mat.rho), 1.0 / 3.0);
+  Rp = pow(0.75 * q[0] / (q[NC - 1] * kPi * mat_rho(mat_of<TAB>(a), q[NC - 2])), 1.0 / 3.0);
   slip = norm3(gp[1] - q[1], gp[2] - q[2], gp[3] - q[3]);
   Re = 2.0 * gp[0] * Rp * slip / a.gMu[g];
 }
 
+// The cell's ep at surface temperature Ts: the material's, the per-cell Yinf, the
+// table's Lv and Cpv at Ts (Mod_Sources.f90:517-519) and its psat (kPsatX)
+template <bool TAB = true> ICE_DEV void cell_ep(const FamArgs& a, long long g, double Ts, double* ep) {
+  const auto& mt = mat_of<TAB>(a);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) ep[m] = mt.ep[m];
+  if (a.var_yinf) ep[kYinf] = a.yinf[g];
+  if (mt.flags & kUseLv) ep[kLv] = prop_value(mt, kPropLv, Ts);
+  if (mt.flags & kUseCpv) ep[kCpv] = prop_value(mt, kPropCpv, Ts);
+  ep[kPsatX] = (mt.flags & kUsePsat) ? prop_value(mt, kPropPsat, Ts) : __longlong_as_double(0x7FF8000000000000ll);
+}
+
 // seed > 0: the interface solve starts from seed x (equilibrium rate), see evaporation_d
-template <int M>
+template <int M, bool TAB = true>
 ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double* src, double& tau, float seed = 0.0f) {
   constexpr int NC = Closure<M>::NC;
   const long long N = a.N;
@@ -156,8 +177,8 @@ ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double*
 #pragma unroll
   for (int m = 0; m < 5; ++m) gp[m] = a.gas[(long long)m * N + g];
   double R = a.gR[g], gam = a.gG[g], gk = a.gK[g], mu = a.gMu[g];
-  const MatDev& mt = a.mat;
-  double rho_mat = mt.rho;
+  const auto& mt = mat_of<TAB>(a);
+  double rho_mat = mat_rho(mt, q[NC - 2]);
   double Rp = pow(0.75 * q[0] / (q[NC - 1] * kPi * rho_mat), 1.0 / 3.0);
   double slip = norm3(gp[1] - q[1], gp[2] - q[2], gp[3] - q[3]);
   double Re = 2.0 * gp[0] * Rp * slip / mu;
@@ -175,10 +196,8 @@ ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double*
   double mdot = 0.0, Qevap = 0.0;
   bool ovr = false;
   if (mt.evap != 0) {
-    double ep[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) ep[m] = mt.ep[m];
-    if (a.var_yinf) ep[kYinf] = a.yinf[g];
+    double ep[kNepCell];
+    cell_ep<TAB>(a, g, Ts, ep);
     evaporation_d(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, mt.evap, mt.intf, ep, mdot, Qevap, ovr, seed,
                   LkStats{a.lk_stats});
   }
@@ -187,7 +206,7 @@ ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double*
     Qconv = Qconv * blowing_factor_d(gam, R, mu, gk, rho_mat, 2.0 * Rp, 4.0 / 3.0 * kPi * (Rp * Rp * Rp) * rho_mat, mdot);
   F[0] = -mdot * q[NC - 1];
   F[4] = Qconv * q[NC - 1];
-  F[4] = F[4] + mt.emiss * kSigmaSB * 4.0 * kPi * Rp * Rp * q[NC - 1] * (pow(gp[4], 4.0) - pow(Ts, 4.0));
+  F[4] = F[4] + mat_emiss(mt, Ts) * kSigmaSB * 4.0 * kPi * Rp * Rp * q[NC - 1] * (pow(gp[4], 4.0) - pow(Ts, 4.0));
   if (Cd == 0.0) tau = __longlong_as_double(0x7FF0000000000000ll);
   else tau = 8.0 * rho_mat * Rp / (3.0 * gp[0] * Cd * slip + kTauToll);
   F[5] = tau;

@@ -28,7 +28,7 @@ constexpr int kMlpLds = kMlpWaveLds * (kThreads / 64);
 // for the cell, clamped to [1e-2, 1].  With 1, every thread of the workgroup
 // calls this (the wave's 64 lanes feed one MLP pass on the MFMA units, skipped
 // when no lane's cell needs the solve).
-template <int M>
+template <int M, bool TAB = true>
 ICE_DEV float stage_seed(const FamArgs& a, bool has_cell, long long g, uint16_t* lds) {
   constexpr int NC = Closure<M>::NC;
   if (a.mat.surrogate == 2) return 1.0f;
@@ -40,15 +40,13 @@ ICE_DEV float stage_seed(const FamArgs& a, bool has_cell, long long g, uint16_t*
     load_cell<NC>(a.cur, a.N, g, q);
 #pragma unroll
     for (int m = 0; m < 5; ++m) gp[m] = a.gas[(long long)m * a.N + g];
-    double ep[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) ep[m] = a.mat.ep[m];
-    if (a.var_yinf) ep[kYinf] = a.yinf[g];
+    double ep[kNepCell];
+    cell_ep<TAB>(a, g, q[NC - 2], ep);
     double p;
     need = lk_needed(gp[0], gp[4], a.gR[g], q[NC - 2], a.mat.evap, a.mat.intf, ep, p);
     if (need) {
       double Rp, slip, Re;
-      drop_state<M>(a, g, q, gp, Rp, slip, Re);
+      drop_state<M, TAB>(a, g, q, gp, Rp, slip, Re);
       mlp_features(gp[4], p, ep[kYinf], q[NC - 2], 2.0 * Rp, Re, f);
     }
   }
@@ -294,7 +292,7 @@ __global__ void __launch_bounds__(kThreads) k_ghost1(FamArgs a) {
           gh[0] = r.massflux / fabs(vn);
           gh[NC - 2] = r.temperature;
         }
-        gh[NC - 1] = gh[0] / a.mat.rho / (4.0 / 3.0 * kPi * (r.radius * r.radius * r.radius));
+        gh[NC - 1] = gh[0] / mat_rho(a.mat, gh[NC - 2]) / (4.0 / 3.0 * kPi * pow(r.radius, 3.0));
         if constexpr (M == 1) gh[4] = 1e-6;
         if constexpr (M == 2) {
           gh[4] = 1e-6;
@@ -501,16 +499,35 @@ __global__ void __launch_bounds__(kThreads) k_bcell(FamArgs a) {
   for (int v = 0; v < NC; ++v) a.bsum[(long long)c * NC + v] = acc[v];
 }
 
+// The source handed to the coupler: the last stage's (keep_source 1), or the
+// step's, each stage's weighted as the update weighs its residual (keep_source 2:
+// add_stage_source / RK_accumulate, Mod_Explicit.f90:257-281, Lib_RK.f90:33-44);
+// the relaxation time of the last stage either way
+template <int NC> ICE_DEV void keep_stage_source(const FamArgs& a, long long g, const double* src, double tau) {
+  if (a.last_stage) a.tau[g] = tau;
+  if (a.keep_source == 2) {
+    const double w = rk_weight(a.nrk, a.srk);
+#pragma unroll
+    for (int v = 0; v < NC; ++v) {
+      double* p = a.src + (long long)v * a.N + g;
+      *p = a.srk == 1 ? src[v] : (*p + src[v]) * w;
+    }
+  } else if (a.last_stage && a.keep_source) {
+    store_cell<NC>(a.src, a.N, g, src);
+  }
+}
+
 // ------------------------------------------------------------------ update
 // state_update_ (Lib_Newstate.f90:152-173) of one cell
-template <int M>
+template <int M, bool TAB = true>
 ICE_DEV void update_cell(const FamArgs& a, const BlkDev& B, int i, int j, int k, long long g, const double* R) {
   constexpr int NC = Closure<M>::NC;
+  const auto& mt = mat_of<TAB>(a);
   double q[NC], co[NC], cu[NC];
   load_cell<NC>(a.old, a.N, g, q);
-  prim2cons<M>(q, co, a.mat);
+  prim2cons<M>(q, co, mt);
   load_cell<NC>(a.cur, a.N, g, q);
-  prim2cons<M>(q, cu, a.mat);
+  prim2cons<M>(q, cu, mt);
   double w = rk_weight(a.nrk, a.srk);
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
@@ -518,7 +535,7 @@ ICE_DEV void update_cell(const FamArgs& a, const BlkDev& B, int i, int j, int k,
     cu[v] = co[v] + rr;
   }
   if (cu[0] < 0.0) atomicAdd(&a.st->n_floored, 1ull);
-  cons2prim<M>(cu, q, a.mat);
+  cons2prim<M>(cu, q, mt);
   store_cell<NC>(a.nxt, a.N, g, q);
   if (!checkprim<M>(q)) {
     if (atomicCAS(&a.st->bad_state, 0, 1) == 0) {
@@ -584,10 +601,7 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage(FamArg
     load_cell<NC>(a.cur, a.N, g, q);
     cell_source<M>(a, g, q, src, tau, seed);
     for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
-    if (a.last_stage) {
-      a.tau[g] = tau;
-      if (a.keep_source) store_cell<NC>(a.src, a.N, g, src);
-    }
+    keep_stage_source<NC>(a, g, src, tau);
   } else {
 #pragma unroll
     for (int v = 0; v < NC; ++v) src[v] = 0.0;
@@ -666,7 +680,8 @@ constexpr int kMaxTileFaces = kThreads;
 // One tile of the stage: the body of k_stage_tiled, also run by the persistent
 // kernel for the tiles it takes from its work queue.  fl is the workgroup's
 // LDS buffer of face fluxes.
-template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
+// TAB = false: the launch serves only families without tabulated properties
+template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM, bool TAB = false>
 ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl, uint16_t* mlds) {
   constexpr int NC = Closure<M>::NC;
   const BlkDev& Bref = a.blk[b];
@@ -707,7 +722,7 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
       bool inside = m >= 1 && m <= n - 1 && li >= 1 && lj >= 1 && lk >= 1 && li <= B.ni && lj <= B.nj && lk <= B.nk;
       if (inside) {
         double q[NC];
-        ok = face_acc<M, S, SD, 2, LIM>(a, gidx(B, li, lj, lk), stride, d, m, n, q) && ok;
+        ok = face_acc<M, S, SD, 2, LIM, TAB>(a, gidx(B, li, lj, lk), stride, d, m, n, q) && ok;
 #pragma unroll
         for (int v = 0; v < NC; ++v) fl[v * kMaxTileFaces + f] = q[v];
       }
@@ -741,18 +756,15 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
   }
   if (!ok) atomicOr(&a.st->bad_recon, 1);
   float seed = 0.0f;
-  if constexpr (COUPLED) seed = stage_seed<M>(a, has_cell, g, mlds);
+  if constexpr (COUPLED) seed = stage_seed<M, TAB>(a, has_cell, g, mlds);
   if (!has_cell) return;
   double src[NC];
   if constexpr (COUPLED) {
     double q[NC], tau;
     load_cell<NC>(a.cur, a.N, g, q);
-    cell_source<M>(a, g, q, src, tau, seed);
+    cell_source<M, TAB>(a, g, q, src, tau, seed);
     for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
-    if (a.last_stage) {
-      a.tau[g] = tau;
-      if (a.keep_source) store_cell<NC>(a.src, a.N, g, src);
-    }
+    keep_stage_source<NC>(a, g, src, tau);
   } else {
 #pragma unroll
     for (int v = 0; v < NC; ++v) src[v] = 0.0;
@@ -765,7 +777,7 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
   if constexpr (WRITE_R) {
     store_cell<NC>(a.R, a.N, g, r);
   } else {
-    update_cell<M>(a, Bref, ci, cj, ck, g, r);
+    update_cell<M, TAB>(a, Bref, ci, cj, ck, g, r);
   }
 }
 

@@ -211,11 +211,10 @@ __global__ __launch_bounds__(kThreads) void k_lk_samples(FamArgs a, int b, float
   int i, j, k;
   cell_of(B, t, i, j, k);
   long long g = gidx(B, i, j, k);
-  double q[NC], gp[5], ep[8];
+  double q[NC], gp[5], ep[kNepCell];
   load_cell<NC>(a.cur, a.N, g, q);
   for (int m = 0; m < 5; ++m) gp[m] = a.gas[(long long)m * a.N + g];
-  for (int m = 0; m < 8; ++m) ep[m] = a.mat.ep[m];
-  if (a.var_yinf) ep[kYinf] = a.yinf[g];
+  cell_ep(a, g, q[NC - 2], ep);
   double p;
   if (!lk_needed(gp[0], gp[4], a.gR[g], q[NC - 2], a.mat.evap, a.mat.intf, ep, p)) return;
   double Rp, slip, Re;
@@ -400,11 +399,30 @@ static int surr_train_impl(ice_gpu_ctx* c, int p, int epochs, int batch, double 
   HIPCHK(hipMemcpy(&ns, dcount, sizeof ns, hipMemcpyDeviceToHost));
   int n = (int)std::min<unsigned>(ns, (unsigned)cap);
   if (n < 64) return fail(c, "[ICE::surrogate_train] %d cells run the interface solve: too few samples", n);
-  // shuffle once on the host (fixed seed), so that mini-batches are spread over the domain
+  // the samples arrive in the order the atomics gave them: put them in a canonical
+  // order (by their values), then shuffle once on the host (fixed seed), so that
+  // mini-batches are spread over the domain and the training repeats bit for bit
   {
     std::vector<float> hx((size_t)n * kMlpIn), hy((size_t)n * kMlpOut);
     HIPCHK(hipMemcpy(hx.data(), dX, hx.size() * sizeof(float), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(hy.data(), dY, hy.size() * sizeof(float), hipMemcpyDeviceToHost));
+    {
+      std::vector<int> order(n);
+      for (int i = 0; i < n; ++i) order[i] = i;
+      auto row = [&](int i, int m) { return m < kMlpIn ? hx[(size_t)i * kMlpIn + m] : hy[(size_t)i * kMlpOut + m - kMlpIn]; };
+      std::sort(order.begin(), order.end(), [&](int a, int b) {
+        for (int m = 0; m < kMlpIn + kMlpOut; ++m)
+          if (row(a, m) != row(b, m)) return row(a, m) < row(b, m);
+        return false;
+      });
+      std::vector<float> sx(hx.size()), sy(hy.size());
+      for (int i = 0; i < n; ++i) {
+        std::copy(&hx[(size_t)order[i] * kMlpIn], &hx[(size_t)order[i] * kMlpIn] + kMlpIn, &sx[(size_t)i * kMlpIn]);
+        std::copy(&hy[(size_t)order[i] * kMlpOut], &hy[(size_t)order[i] * kMlpOut] + kMlpOut, &sy[(size_t)i * kMlpOut]);
+      }
+      hx.swap(sx);
+      hy.swap(sy);
+    }
     uint64_t st = seed ^ 0x5DEECE66Dull;
     for (int i = n - 1; i > 0; --i) {
       int j = (int)(splitmix(st) % (uint64_t)(i + 1));

@@ -53,11 +53,16 @@ module ice_gpu_m
     real(c_double)     :: first_largest = 0, largest = 0
   end type
   integer, parameter, public :: ICE_INFLOW400_LOG = 4096
+  !> columns and flags of ice_gpu_set_properties
+  integer, parameter, public :: ICE_PROP_RHO = 0, ICE_PROP_CP = 1, ICE_PROP_E = 2, ICE_PROP_PSAT = 3, &
+                                ICE_PROP_LV = 4, ICE_PROP_CPV = 5, ICE_PROP_EMISS = 6, ICE_PROP_NCOL = 7
+  integer, parameter, public :: ICE_PROP_RHO_VARIES = 1, ICE_PROP_CS_VARIES = 2, ICE_PROP_USE_PSAT = 4, &
+                                ICE_PROP_USE_LV = 8, ICE_PROP_USE_CPV = 16, ICE_PROP_USE_EMISS = 32
 
   public :: ice_gpu_create, ice_gpu_set_geometry, ice_gpu_set_boundaries, ice_gpu_put_state, ice_gpu_get_state, &
             ice_gpu_put_gas, ice_gpu_put_tau, ice_gpu_step, ice_gpu_residual_norm, ice_gpu_timing, &
             ice_gpu_last_error, ice_gpu_destroy, ice_gpu_error_text, ice_gpu_autotune, ice_gpu_inflow400_lines, &
-            ice_gpu_inflow400_totals
+            ice_gpu_inflow400_totals, ice_gpu_set_properties
 
   interface
     integer(c_int) function ice_gpu_create(device, cfg, fams, nb, dims, ctx) bind(C, name='ice_gpu_create')
@@ -130,6 +135,13 @@ module ice_gpu_m
       integer(c_int), value :: steps
       integer(c_int32_t) :: best(4)
       real(c_double) :: ms(16)
+    end function
+    integer(c_int) function ice_gpu_set_properties(ctx, p, t_min, n, cols, flags) bind(C, name='ice_gpu_set_properties')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: p, t_min, n
+      real(c_double) :: cols(*)
+      integer(c_int32_t), value :: flags
     end function
     integer(c_int) function ice_gpu_inflow400_lines(ctx, lines, max, n, dropped) bind(C, name='ice_gpu_inflow400_lines')
       import :: c_int, c_int32_t, c_int64_t, c_ptr, ice_inflow400_line_t

@@ -338,30 +338,108 @@ contains
 
   !> A constant property table (Temperature, Cp, Density, Enthalpy rows); the
   !> device path holds one density and heat capacity per material.
-  subroutine read_material(path, rho_mat, cs)
+  !> The property table INPUT/part-properties.dat (Load_Table.f90:57-147): the
+  !> VARIABLES header names the columns (Temperature first, then Cp, Density and
+  !> Enthalpy or Enthalpy_abs in any order, other columns ignored), one POINT row per
+  !> integer kelvin.  Returns the first row's density and cp, and the device columns
+  !> of ice_gpu_set_properties: density, cp, e = h - hOff (table_h_offset, :535-543),
+  !> with the rho_varies / cs_varies flags.
+  subroutine read_material(path, rho_mat, cs, t_min, cols, flags)
     character(len=*), intent(in) :: path
     real(R8), intent(out) :: rho_mat, cs
-    character(len=512) :: line
-    real(R8) :: r(4)
-    integer :: u, ios, n
+    integer, intent(out) :: t_min, flags
+    real(R8), allocatable, intent(out) :: cols(:,:)
+    character(len=1024) :: line
+    character(len=64) :: names(32)
+    real(R8), allocatable :: rows(:,:), tmp(:,:)
+    real(R8) :: h_off, T1
+    integer :: u, ios, n, nv, it, icp, irho, ih, m
     open(newunit=u, file=path, status='old', action='read', iostat=ios)
     if (ios /= 0) call refuse('Load_Table', 'cannot open '//trim(path))
+    nv = 0
     n = 0
+    allocate(rows(32, 64))
     do
       read(u, '(A)', iostat=ios) line
       if (ios /= 0) exit
-      read(line, *, iostat=ios) r
+      if (index(line, 'VARIABLES') > 0) then
+        call header_names(line, names, nv)
+        cycle
+      end if
+      if (nv == 0) cycle
+      read(line, *, iostat=ios) rows(1:nv, min(n + 1, size(rows, 2)))
       if (ios /= 0) cycle
       n = n + 1
-      if (n == 1) then
-        cs = r(2)
-        rho_mat = r(3)
-      else if (r(2) /= cs .or. r(3) /= rho_mat) then
-        call refuse('Load_Table', trim(path)//': the device path takes constant Cp and density columns')
+      if (n == size(rows, 2)) then
+        allocate(tmp(32, 2 * n))
+        tmp(:, 1:n) = rows
+        call move_alloc(tmp, rows)
       end if
     end do
     close(u)
-    if (n == 0) call refuse('Load_Table', trim(path)//' holds no property row')
+    if (nv == 0) call refuse('Load_Table', trim(path)//': no VARIABLES header')
+    if (n < 2) call refuse('Load_Table', trim(path)//': a property table needs at least two rows')
+    it = column('Temperature'); icp = column('Cp'); irho = column('Density'); ih = column('Enthalpy')
+    if (ih == 0) ih = column('Enthalpy_abs')
+    if (it /= 1) call refuse('Load_Table', trim(path)//': Temperature must be the first column')
+    if (icp == 0 .or. irho == 0 .or. ih == 0) call refuse('Load_Table', trim(path)//': Cp, Density and Enthalpy needed')
+    t_min = nint(rows(1, 1))
+    if (t_min < 0) call refuse('Load_Table', trim(path)//': a temperature below 0 K')
+    do m = 1, n
+      if (abs(rows(1, m) - real(t_min + m - 1, R8)) > 1.e-6_R8) &
+        call refuse('Load_Table', trim(path)//': rows not on consecutive integer kelvins')
+      if (rows(irho, m) <= 0._R8 .or. rows(icp, m) <= 0._R8) &
+        call refuse('Load_Table', trim(path)//': a density or cp that is not positive')
+      if (m > 1) then
+        if (rows(ih, m) <= rows(ih, m - 1)) call refuse('Load_Table', trim(path)//': an enthalpy that does not increase')
+      end if
+    end do
+    T1 = rows(1, 1)
+    if (all(rows(icp, 1:n) == rows(icp, 1))) then
+      h_off = rows(ih, 1) - rows(icp, 1) * T1
+    else
+      h_off = rows(ih, 1) - T1 * (rows(ih, 2) - rows(ih, 1))
+    end if
+    allocate(cols(n, 0:ICE_PROP_NCOL - 1))
+    cols = 0._R8
+    cols(:, ICE_PROP_RHO) = rows(irho, 1:n)
+    cols(:, ICE_PROP_CP) = rows(icp, 1:n)
+    cols(:, ICE_PROP_E) = rows(ih, 1:n) - h_off
+    flags = 0
+    if (any(rows(irho, 1:n) /= rows(irho, 1))) flags = ior(flags, ICE_PROP_RHO_VARIES)
+    if (any(rows(icp, 1:n) /= rows(icp, 1))) flags = ior(flags, ICE_PROP_CS_VARIES)
+    rho_mat = rows(irho, 1)
+    cs = rows(icp, 1)
+  contains
+    integer function column(name)
+      character(len=*), intent(in) :: name
+      integer :: k
+      column = 0
+      do k = 1, nv
+        if (trim(names(k)) == name) column = k
+      end do
+    end function
+  end subroutine
+
+  !> The quoted names of a VARIABLES line
+  subroutine header_names(line, names, nv)
+    character(len=*), intent(in) :: line
+    character(len=64), intent(out) :: names(:)
+    integer, intent(out) :: nv
+    integer :: a, b
+    nv = 0
+    a = index(line, '"')
+    do while (a > 0 .and. nv < size(names))
+      b = index(line(a + 1:), '"')
+      if (b == 0) exit
+      nv = nv + 1
+      names(nv) = line(a + 1:a + b - 1)
+      a = a + b + 1
+      if (a > len_trim(line)) exit
+      b = index(line(a:), '"')
+      if (b == 0) exit
+      a = a + b - 1
+    end do
   end subroutine
 
   ! ------------------------------------------------------------------ BC

@@ -18,7 +18,8 @@ program ice_driver
   type(ice_family_t), allocatable :: fams(:)
   type(ice_status_t) :: st
   integer(c_int32_t), allocatable :: reci(:,:), donor(:,:), dims(:)
-  real(R8), allocatable :: recd(:,:), weight(:), fam(:)
+  real(R8), allocatable :: recd(:,:), weight(:), fam(:), prop_cols(:,:)
+  integer :: t_min, prop_flags
   real(R8) :: rho_mat, cs, total(5), step_ms, kernel_ms, tile_ms(16)
   integer(c_int32_t) :: tile(4)
   integer(c_int64_t) :: launches
@@ -41,7 +42,7 @@ program ice_driver
   case default; nc = 12; names = 'rho_p1 u_p1 v_p1 w_p1 P11_p1 P12_p1 P13_p1 P22_p1 P23_p1 P33_p1 T_p1 n_p1'
   end select
   call read_ic(trim(dir)//'/INPUT/part-ic.tec', nc, blocks)
-  call read_material(trim(dir)//'/INPUT/part-properties.dat', rho_mat, cs)
+  call read_material(trim(dir)//'/INPUT/part-properties.dat', rho_mat, cs, t_min, prop_cols, prop_flags)
   nb = size(blocks)
   ndir = 0
   do n = 1, 3
@@ -59,6 +60,8 @@ program ice_driver
   end do
 
   call check(ice_gpu_create(device, cfg, fams, nb, dims, ctx), 'create')
+  call check(ice_gpu_set_properties(ctx, 1, t_min, size(prop_cols, 1), prop_cols, int(prop_flags, c_int32_t)), &
+             'set_properties')
   do b = 1, nb
     call check(ice_gpu_set_geometry(ctx, b, blocks(b)%vol, blocks(b)%dl, blocks(b)%m, blocks(b)%fi, &
                                     blocks(b)%fj, blocks(b)%fk), 'set_geometry')

@@ -101,6 +101,11 @@ SIGNATURES = {
     "ice_gpu_surrogate_set_params": [C.c_void_p, C.c_int, _FP],
     "ice_gpu_set_lk_stats": [C.c_void_p, C.c_int],
     "ice_gpu_lk_stats": [C.c_void_p, C.c_int, _LP],
+    "ice_gpu_restrict": [C.c_void_p, C.c_void_p],
+    "ice_gpu_prolong": [C.c_void_p, C.c_void_p],
+    "ice_gpu_set_res_threshold": [C.c_void_p, C.c_double],
+    "ice_gpu_res_stop": [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)],
+    "ice_gpu_set_properties": [C.c_void_p, C.c_int, C.c_int, C.c_int, _DP, C.c_int32],
     "ice_gpu_inflow400_lines": [C.c_void_p, C.POINTER(Inflow400Line), C.c_int, C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int64)],
     "ice_gpu_inflow400_totals": [C.c_void_p, C.POINTER(Inflow400Total), C.c_int, C.POINTER(C.c_int32)],

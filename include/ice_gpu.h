@@ -144,8 +144,20 @@ int  ice_gpu_put_tau(ice_gpu_ctx* ctx, int b, int p, const double* tau);
  * Returns non-zero when a step hit a fatal state (st->bad_state / bad_recon). */
 int  ice_gpu_step(ice_gpu_ctx* ctx, int nsteps, ice_status_t* st);
 
-/* 1: the last stage of every step keeps its source for ice_gpu_get_source. */
+/* 1: the last stage of every step keeps its source for ice_gpu_get_source.
+ * 2: the step's source, each stage's weighted as the update weighs its residual
+ *    (add_stage_source / hand_step_source, Mod_Explicit.f90:257-300): what a
+ *    two-way coupled run hands the gas (exchange_rate, Mod_Exchange.f90:83-103). */
 int  ice_gpu_set_keep_source(ice_gpu_ctx* ctx, int on);
+
+/* res-threshold of a steady run (Mod_Explicit.f90:196-225): each ice_gpu_step then
+ * computes the residual norm after every step and ends early, after the step whose
+ * density residual is at most threshold times the first non-zero one of the run.
+ * 0 (the default) never stops on the residual; time-accurate runs never do.  Setting
+ * it restarts the reference. */
+int  ice_gpu_set_res_threshold(ice_gpu_ctx* ctx, double threshold);
+/* Whether the last ice_gpu_step ended on the residual, and the reference residual. */
+int  ice_gpu_res_stop(ice_gpu_ctx* ctx, int32_t* stopped, double* reference);
 
 /* Last stage's source (nc,ni,nj,nk) and relaxation time (ni,nj,nk) of family p. */
 int  ice_gpu_get_source(ice_gpu_ctx* ctx, int b, int p, double* source, double* tau);
@@ -207,6 +219,34 @@ int  ice_gpu_surrogate_set_params(ice_gpu_ctx* ctx, int p, const float* params);
  * map evaluations, fallbacks to the plain solve and the largest evaluation count. */
 int  ice_gpu_set_lk_stats(ice_gpu_ctx* ctx, int on);
 int  ice_gpu_lk_stats(ice_gpu_ctx* ctx, int p, int64_t* counts);
+
+/* ---- grid sequencing (Mod_Multigrid.f90:93-131, Lib_Multigrid.f90:40-191): two
+ * contexts of one run one level apart on the same device, with the same families
+ * and blocks; each coarse block has half the fine block's cells in every direction
+ * (a flat block keeps nk = 1).  Restriction sets the coarse state to the fine
+ * one's conservative state averaged over the coarse cell's fine cells, weighted by
+ * volume; prolongation sets the fine state by trilinear interpolation of the
+ * coarse primitives (27/64, 9/64, 3/64, 1/64; flat: 9/16, 3/16, 1/16), neighbours
+ * clamped to the block.  The interiors only: the next stage fills the ghosts. */
+int  ice_gpu_restrict(ice_gpu_ctx* fine, ice_gpu_ctx* coarse);
+int  ice_gpu_prolong(ice_gpu_ctx* coarse, ice_gpu_ctx* fine);
+
+/* ---- tabulated material properties (Lib_Properties.f90:18-250, the table that
+ * Load_Table.f90:131-147 loads).  Family p's material gets n rows on the integer
+ * kelvins T_min .. T_min+n-1; cols holds ICE_PROP_NCOL columns of n values each:
+ * density, cp, energy e = h - hOff, Psat, Lv, Cpv, emissivity.  Each property is
+ * linear between the nodes and takes its end value outside them; e is extended
+ * linearly beyond both ends and inverted by bisection for T.  flags says which
+ * columns the path takes, as Load_Table decides: density varies (else the first
+ * row's), cp varies (the closures' energy is then e(T)), and Psat, Lv, Cpv and
+ * emissivity replace Clausius-Clapeyron and the constant values.  The table's first
+ * rows of density and cp replace the family's rho_mat and cs.  Call before the
+ * first step; it replaces an earlier table of the family. */
+enum { ICE_PROP_RHO = 0, ICE_PROP_CP, ICE_PROP_E, ICE_PROP_PSAT, ICE_PROP_LV, ICE_PROP_CPV, ICE_PROP_EMISS,
+       ICE_PROP_NCOL };
+enum { ICE_PROP_RHO_VARIES = 1, ICE_PROP_CS_VARIES = 2, ICE_PROP_USE_PSAT = 4, ICE_PROP_USE_LV = 8,
+       ICE_PROP_USE_CPV = 16, ICE_PROP_USE_EMISS = 32 };
+int  ice_gpu_set_properties(ice_gpu_ctx* ctx, int p, int T_min, int n, const double* cols, int32_t flags);
 
 /* ---- particles entering through extrapolation faces (400): count_400_inflow and
  * report_400_inflow (src/lib/diagnostic/Mod_Inflow400.f90:52-171).  Every stage

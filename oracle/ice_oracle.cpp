@@ -29,6 +29,7 @@ static const int GUIDE[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0
 struct Family {
   int model, solver, nc, nbase, nfrozen, frozen[4];
   Mat mat;
+  std::vector<double> prop;   // the material's property table (mat.tab points here)
   int surrogate = 0;          // 1: network seed, 2: equilibrium seed
   std::vector<float> mlp;     // the network's parameters (surrogate = 1)
 };
@@ -42,6 +43,7 @@ struct Rec {
 
 struct Phase {  // ICE_cond_phase_type (Advanced_Types_m.f90:46-84), padded layout
   std::vector<double> prim, prim_old, residual, source, beta, dt, tau;
+  std::vector<double> source_step;  // add_stage_source's running sum (Mod_Explicit.f90:257-281)
   std::vector<char> inj;
 };
 
@@ -84,6 +86,9 @@ struct orc_ctx {
   std::vector<std::vector<double>> i400_first_x, i400_largest_x, i400_line_x;
   std::vector<ice_inflow400_line_t> i400_lines;
   int64_t i400_logged = 0;
+  int keep_source = 0;                       // 2: hand the step's source (hand_step_source)
+  double res_threshold = 0.0, res_reference = 0.0;
+  int res_stopped = 0;
   std::string err;
 };
 
@@ -411,8 +416,9 @@ void ghost_record(orc_ctx* c, int n) {
           gh[0] = r.massflux / std::fabs(vn);
           gh[nb - 2] = r.temperature;
         }
-        // N from rho and the inlet radius, with the material density (constant)
-        gh[nb - 1] = gh[0] / F.mat.rho / (4.0 / 3.0 * PI * (r.radius * r.radius * r.radius));
+        // N from rho and the inlet radius, with the material density at the inlet
+        // temperature (Lib_Ghost.f90:230-232; radius**3._I4 is a real power)
+        gh[nb - 1] = gh[0] / mat_rho(F.mat, gh[nb - 2]) / (4.0 / 3.0 * PI * pow_libm(r.radius, 3.0));
         if (F.model == IG) gh[4] = 1e-6;
         else if (F.model == AG) { gh[4] = 1e-6; gh[7] = 1e-6; gh[9] = 1e-6; }
       }
@@ -656,7 +662,7 @@ void compute_flux(orc_ctx* c, int p) {
 void source_cell(orc_ctx* c, const Family& F, const double* q, const double* gp, double R, double gam, double gk,
                  double mu, double yinf, double* tau, double* src) {
   int n = F.nbase;
-  double rho_mat = F.mat.rho;
+  double rho_mat = mat_rho(F.mat, q[n - 2]);
   double Rp = std::pow(0.75 * q[0] / (q[n - 1] * PI * rho_mat), 1.0 / 3.0);
   double slip = norm2_3(gp[1] - q[1], gp[2] - q[2], gp[3] - q[3]);
   double Re = 2.0 * gp[0] * Rp * slip / mu;
@@ -674,6 +680,15 @@ void source_cell(orc_ctx* c, const Family& F, const double* q, const double* gp,
   double ep[8];
   for (int m = 0; m < 8; ++m) ep[m] = F.mat.ep[m];
   if (c->cfg.var_yinf) ep[EP_YINF] = yinf;
+  // the table's Lv, Cpv and Psat at the surface temperature (Mod_Sources.f90:517-535)
+  if (F.mat.flags & ICE_PROP_USE_LV) ep[EP_LV] = table_value(F.mat, ICE_PROP_LV, Ts);
+  if (F.mat.flags & ICE_PROP_USE_CPV) ep[EP_CPV] = table_value(F.mat, ICE_PROP_CPV, Ts);
+  double psat_tab = 0.0;
+  const double* psat_ext = nullptr;
+  if (F.mat.flags & ICE_PROP_USE_PSAT) {
+    psat_tab = table_value(F.mat, ICE_PROP_PSAT, Ts);
+    psat_ext = &psat_tab;
+  }
   double seed = 0.0;
   if (F.surrogate == 2) {
     seed = 1.0;
@@ -687,13 +702,13 @@ void source_cell(orc_ctx* c, const Family& F, const double* q, const double* gp,
     seed = r < 1e-2f ? 1e-2f : (r > 1.0f ? 1.0f : r);
   }
   evaporation(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, F.mat.evap, F.mat.intf, ep, mdot, Qevap, ovr, nullptr,
-              seed);
+              seed, psat_ext);
   if (ovr) Qconv = Qevap;
   else if (c->cfg.blow_select == 1)
     Qconv = Qconv * blowing_factor(gam, R, mu, gk, rho_mat, 2.0 * Rp, 4.0 / 3.0 * PI * (Rp * Rp * Rp) * rho_mat, mdot);
   force[0] = -mdot * q[n - 1];
   force[4] = Qconv * q[n - 1];
-  force[4] = force[4] + F.mat.emiss * SIGMA_SB * 4.0 * PI * Rp * Rp * q[n - 1] *
+  force[4] = force[4] + mat_emiss(F.mat, Ts) * SIGMA_SB * 4.0 * PI * Rp * Rp * q[n - 1] *
                             (std::pow(gp[4], 4.0) - std::pow(Ts, 4.0));
   if (Cd == 0.0) *tau = std::numeric_limits<double>::infinity();
   else *tau = 8.0 * rho_mat * Rp / (3.0 * gp[0] * Cd * slip + TAU_TOLL);
@@ -836,7 +851,19 @@ void one_step(orc_ctx* c) {
             for (int i = 1; i <= B.ni; ++i)
               for (int v = 0; v < nc; ++v) ph.residual[B.id(i, j, k) * nc + v] = 0.0;
       }
-      if (cf.coupled) compute_source(c, p);
+      if (cf.coupled) {
+        compute_source(c, p);
+        // add_stage_source with RK_accumulate (Mod_Explicit.f90:257-281, Lib_RK.f90:33-44)
+        if (c->keep_source == 2) {
+          double w = rkc(cf.nrk, srk);
+          for (auto& B : c->blk) {
+            Phase& ph = B.ph[p - 1];
+            if (ph.source_step.size() != ph.source.size()) ph.source_step.assign(ph.source.size(), 0.0);
+            for (size_t m = 0; m < ph.source.size(); ++m)
+              ph.source_step[m] = srk == 1 ? ph.source[m] : (ph.source_step[m] + ph.source[m]) * w;
+          }
+        }
+      }
       {
         const std::vector<int>& g = c->grp[p - 1];
 #pragma omp parallel for schedule(dynamic, 64)
@@ -1127,9 +1154,18 @@ int orc_put_tau(orc_ctx* c, int b, int p, const double* tau) {
 }
 
 int orc_step(orc_ctx* c, int nsteps, ice_status_t* st) {
+  c->res_stopped = 0;
   for (int s = 0; s < nsteps; ++s) {
     one_step(c);
     if (c->bad_state || c->bad_recon) break;
+    // res-threshold of a steady run (Mod_Explicit.f90:196-225)
+    if (c->res_threshold > 0.0 && !c->cfg.time_accurate) {
+      if (c->res_reference == 0.0) c->res_reference = c->restot[0];
+      if (c->res_reference > 0.0 && c->restot[0] <= c->res_threshold * c->res_reference) {
+        c->res_stopped = 1;
+        break;
+      }
+    }
   }
   if (st) {
     st->n_floored = c->n_floored;
@@ -1148,15 +1184,152 @@ int orc_get_source(orc_ctx* c, int b, int p, double* source, double* tau) {
       for (int i = 1; i <= B.ni; ++i) {
         size_t s = (size_t)(i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (k - 1));
         size_t id = B.id(i, j, k);
-        for (int v = 0; v < nc; ++v) source[s * nc + v] = B.ph[p - 1].source[id * nc + v];
+        const std::vector<double>& src = c->keep_source == 2 ? B.ph[p - 1].source_step : B.ph[p - 1].source;
+        for (int v = 0; v < nc; ++v) source[s * nc + v] = src[id * nc + v];
         tau[s] = B.ph[p - 1].tau[id];
       }
+  return 0;
+}
+
+// Check_Multigrid (Lib_Multigrid.f90:12-35) for two contexts one level apart
+static int level_pair(orc_ctx* f, orc_ctx* c) {
+  if (f->cfg.ngroups != c->cfg.ngroups || f->blk.size() != c->blk.size()) {
+    f->err = "[ICE::Multigrid] the levels have different families or blocks";
+    return 1;
+  }
+  for (size_t b = 0; b < f->blk.size(); ++b) {
+    const Blk &F = f->blk[b], &C = c->blk[b];
+    bool ok = F.ni % 2 == 0 && F.nj % 2 == 0 && C.ni * 2 == F.ni && C.nj * 2 == F.nj &&
+              (F.nk == 1 ? C.nk == 1 : (F.nk % 2 == 0 && C.nk * 2 == F.nk));
+    if (!ok) {
+      f->err = " [ERROR] [ICE::Check_Multigrid] block not divisible by 2^(MGL-1)";
+      return 1;
+    }
+  }
+  return 0;
+}
+
+// fine2coarse_prim (Lib_Multigrid.f90:40-109)
+int orc_restrict(orc_ctx* fine, orc_ctx* coarse) {
+  if (level_pair(fine, coarse)) return 1;
+  for (size_t b = 0; b < fine->blk.size(); ++b) {
+    const Blk& F = fine->blk[b];
+    Blk& Cb = coarse->blk[b];
+    for (int p = 1; p <= fine->cfg.ngroups; ++p) {
+      const Family& fam = fine->fam[p - 1];
+      const int nc = fam.nc;
+      // fCons of every fine cell
+      std::vector<double> fcons((size_t)F.ni * F.nj * F.nk * nc);
+      auto fid = [&](int i, int j, int k) { return ((size_t)(i - 1) + (size_t)F.ni * ((j - 1) + (size_t)F.nj * (k - 1))) * nc; };
+      for (int k = 1; k <= F.nk; ++k)
+        for (int j = 1; j <= F.nj; ++j)
+          for (int i = 1; i <= F.ni; ++i)
+            p2c(fam.model, &F.ph[p - 1].prim[F.id(i, j, k) * nc], &fcons[fid(i, j, k)], fam.mat);
+      std::vector<double> cc(nc), q(nc);
+      for (int k = 1; k <= Cb.nk; ++k)
+        for (int j = 1; j <= Cb.nj; ++j)
+          for (int i = 1; i <= Cb.ni; ++i) {
+            int i2 = 2 * i, j2 = 2 * j, k2 = 2 * k, i2d = i2 - 1, j2d = j2 - 1, k2d = k2 - 1;
+            auto term = [&](int a, int bb, int cc3, int v) { return fcons[fid(a, bb, cc3) + v] * F.vol[F.id(a, bb, cc3)]; };
+            for (int v = 0; v < nc; ++v) {
+              if (F.nk == 1) {
+                cc[v] = term(i2d, j2d, 1, v) + term(i2, j2d, 1, v) + term(i2d, j2, 1, v) + term(i2, j2, 1, v);
+              } else {
+                cc[v] = term(i2d, j2d, k2d, v) + term(i2, j2d, k2d, v) + term(i2d, j2, k2d, v) + term(i2, j2, k2d, v) +
+                        term(i2d, j2d, k2, v) + term(i2, j2d, k2, v) + term(i2d, j2, k2, v) + term(i2, j2, k2, v);
+              }
+            }
+            for (int v = 0; v < nc; ++v) cc[v] = cc[v] / Cb.vol[Cb.id(i, j, k)];
+            c2p(fam.model, cc.data(), &Cb.ph[p - 1].prim[Cb.id(i, j, k) * nc], coarse->fam[p - 1].mat);
+          }
+    }
+  }
+  return 0;
+}
+
+// coarse2fine_prim (Lib_Multigrid.f90:113-191)
+int orc_prolong(orc_ctx* coarse, orc_ctx* fine) {
+  if (level_pair(fine, coarse)) return 1;
+  for (size_t b = 0; b < fine->blk.size(); ++b) {
+    Blk& F = fine->blk[b];
+    const Blk& Cb = coarse->blk[b];
+    double co[8];
+    if (F.nk == 1) {
+      double a1 = 9.0 / 16.0, a2 = 3.0 / 16.0, a3 = 1.0 / 16.0, a4 = 0.0;
+      double t[8] = {a1, a2, a2, a4, a3, a4, a4, a4};
+      std::copy(t, t + 8, co);
+    } else {
+      double a1 = 27.0 / 64.0, a2 = 9.0 / 64.0, a3 = 3.0 / 64.0, a4 = 1.0 / 64.0;
+      double t[8] = {a1, a2, a2, a2, a3, a3, a3, a4};
+      std::copy(t, t + 8, co);
+    }
+    static const int MASK[8][3] = {{1, 2, 3}, {4, 2, 3}, {1, 5, 3}, {4, 5, 3}, {1, 2, 6}, {4, 2, 6}, {1, 5, 6}, {4, 5, 6}};
+    for (int p = 1; p <= fine->cfg.ngroups; ++p) {
+      const int nc = fine->fam[p - 1].nc;
+      const std::vector<double>& cp = Cb.ph[p - 1].prim;
+      std::vector<double>& fp = F.ph[p - 1].prim;
+      auto C = [&](int i, int j, int k, int v) { return cp[Cb.id(i, j, k) * nc + v]; };
+      for (int k = 1; k <= Cb.nk; ++k)
+        for (int j = 1; j <= Cb.nj; ++j)
+          for (int i = 1; i <= Cb.ni; ++i) {
+            int i2 = 2 * i, j2 = 2 * j, k2 = 2 * k, i2d = i2 - 1, j2d = j2 - 1, k2d = k2 - 1;
+            int im = std::max(1, i - 1), ip = std::min(Cb.ni, i + 1);
+            int jm = std::max(1, j - 1), jp = std::min(Cb.nj, j + 1);
+            int km = std::max(1, k - 1), kp = std::min(Cb.nk, k + 1);
+            if (F.nk == 1) { k2 = 1; k2d = 1; }
+            int id[6] = {im, jm, km, ip, jp, kp};
+            int counter = 0;
+            for (int kk = k2d; kk <= k2; ++kk)
+              for (int jj = j2d; jj <= j2; ++jj)
+                for (int ii = i2d; ii <= i2; ++ii) {
+                  const int* m = MASK[counter];
+                  int a = id[m[0] - 1], bb = id[m[1] - 1], cc3 = id[m[2] - 1];
+                  for (int v = 0; v < nc; ++v)
+                    fp[F.id(ii, jj, kk) * nc + v] = co[0] * C(i, j, k, v) + co[1] * C(a, j, k, v) + co[2] * C(i, bb, k, v) +
+                                                    co[3] * C(i, j, cc3, v) + co[4] * C(a, bb, k, v) +
+                                                    co[5] * C(a, j, cc3, v) + co[6] * C(i, bb, cc3, v) +
+                                                    co[7] * C(a, bb, cc3, v);
+                  counter += 1;
+                }
+          }
+    }
+  }
+  return 0;
+}
+
+int orc_set_keep_source(orc_ctx* c, int on) {
+  c->keep_source = on == 2 ? 2 : (on ? 1 : 0);
+  return 0;
+}
+int orc_set_res_threshold(orc_ctx* c, double threshold) {
+  c->res_threshold = threshold;
+  c->res_reference = 0.0;
+  c->res_stopped = 0;
+  return 0;
+}
+int orc_res_stop(orc_ctx* c, int32_t* stopped, double* reference) {
+  if (stopped) *stopped = c->res_stopped;
+  if (reference) *reference = c->res_reference;
   return 0;
 }
 
 int orc_residual_norm(orc_ctx* c, double* total, double* fam) {
   for (int m = 0; m < 5; ++m) total[m] = c->restot[m];
   for (size_t m = 0; m < c->resfam.size(); ++m) fam[m] = c->resfam[m];
+  return 0;
+}
+
+int orc_set_properties(orc_ctx* c, int p, int t_min, int n, const double* cols, int32_t flags) {
+  if (p < 1 || p > c->cfg.ngroups) { c->err = "[ICE::set_properties] family out of range"; return 1; }
+  if (n < 2 || t_min < 0) { c->err = "[ICE::Load_Table] a property table needs 2 rows on T >= 0"; return 1; }
+  Family& F = c->fam[p - 1];
+  F.prop.assign(cols, cols + (size_t)ICE_PROP_NCOL * n);
+  F.mat.tab = F.prop.data();
+  F.mat.tmin = t_min;
+  F.mat.tmax = t_min + n - 1;
+  F.mat.flags = flags;
+  F.mat.rho = cols[(size_t)ICE_PROP_RHO * n];
+  F.mat.cs = cols[(size_t)ICE_PROP_CP * n];
   return 0;
 }
 
@@ -1230,9 +1403,22 @@ void orc_destroy(orc_ctx* c) { delete c; }
 extern "C" {
 static Mat g_mat;
 static int g_model = IG;
+static std::vector<double> g_tab;
 void orc_fn_setup(int model, double rho, double cs, double lv, double emiss) {
   g_model = model; g_mat.rho = rho; g_mat.cs = cs; g_mat.lv = lv; g_mat.emiss = emiss;
+  g_mat.tab = nullptr; g_mat.flags = 0;
 }
+// a property table for the per-function entry points (ice_gpu_set_properties' layout)
+void orc_fn_setup_table(int t_min, int n, const double* cols, int flags) {
+  g_tab.assign(cols, cols + (size_t)ICE_PROP_NCOL * n);
+  g_mat.tab = g_tab.data(); g_mat.tmin = t_min; g_mat.tmax = t_min + n - 1; g_mat.flags = flags;
+  g_mat.rho = cols[(size_t)ICE_PROP_RHO * n]; g_mat.cs = cols[(size_t)ICE_PROP_CP * n];
+}
+double orc_fn_table_value(int col, double T) { return table_value(g_mat, col, T); }
+double orc_fn_mat_e(double T) { return mat_e(g_mat, T); }
+double orc_fn_mat_T_from_e(double e) { return mat_T_from_e(g_mat, e); }
+double orc_fn_mat_T_from_rhoe(double rhoe, double rho, double eps) { return mat_T_from_rhoe(g_mat, rhoe, rho, eps); }
+double orc_fn_mat_rho(double T) { return mat_rho(g_mat, T); }
 void orc_fn_p2c(const double* q, double* c) { p2c(g_model, q, c, g_mat); }
 void orc_fn_c2p(const double* c, double* q) { c2p(g_model, c, q, g_mat); }
 int orc_fn_check(const double* q) { return check_prim(g_model, q) ? 1 : 0; }
@@ -1286,6 +1472,14 @@ void orc_fn_evaporation(double rhog, double Tg, double gam, double Rg, double mu
                         int* iters) {
   bool o = false;
   evaporation(rhog, Tg, gam, Rg, mug, kg, Tp, dp, Re, evap, intf, ep, *mdot, *qdot, o, iters);
+  *ovr = o ? 1 : 0;
+}
+// the same with the property table's saturation pressure (psatExt)
+void orc_fn_evaporation_psat(double rhog, double Tg, double gam, double Rg, double mug, double kg, double Tp, double dp,
+                             double Re, int evap, int intf, const double* ep, double psat, double* mdot, double* qdot,
+                             int* ovr) {
+  bool o = false;
+  evaporation(rhog, Tg, gam, Rg, mug, kg, Tp, dp, Re, evap, intf, ep, *mdot, *qdot, o, nullptr, 0.0, &psat);
   *ovr = o ? 1 : 0;
 }
 double orc_fn_blowing(double gam, double Rg, double mug, double kg, double rhop, double dp, double mp, double mdot) {

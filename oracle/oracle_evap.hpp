@@ -219,8 +219,9 @@ static inline void lk_map(const EvapState& s, double XsEq, double LKd2, double m
 // G(m) = map(m) - m decreases strictly on [md_eq, 0], G(md_eq) >= 0 > G(0), so
 // that interval brackets its root; secant steps from the seed m0 and its
 // Picard image, kept inside the bracket (else its midpoint), converged by
-// lkCorrection's own test; false after SEED_EVALS evaluations.
-static const int SEED_EVALS = 16;
+// lkCorrection's own test; every second step that has not halved the bracket
+// is replaced by a bisection; false after SEED_EVALS evaluations.
+static const int SEED_EVALS = 64;
 static inline bool lk_secant(const EvapState& s, double XsEq, double LKd2, double md_eq, double m0, double& md,
                              double& Qd, bool& ovr, int& evals) {
   double lo = md_eq, hi = 0.0;
@@ -237,6 +238,8 @@ static inline bool lk_secant(const EvapState& s, double XsEq, double LKd2, doubl
   else hi = m0;
   double m1 = f0;
   if (!(m1 > lo && m1 < hi)) m1 = 0.5 * (lo + hi);
+  double wref = hi - lo;
+  int since = 0;
   while (evals < SEED_EVALS) {
     lk_map(s, XsEq, LKd2, m1, f1, Q, o);
     ++evals;
@@ -250,16 +253,21 @@ static inline bool lk_secant(const EvapState& s, double XsEq, double LKd2, doubl
     double den = g1 - g0;
     double m2 = den != 0.0 ? m1 - g1 * (m1 - m0) / den : 0.5 * (lo + hi);
     if (!(m2 > lo && m2 < hi)) m2 = 0.5 * (lo + hi);
+    if (++since == 2) {
+      if (hi - lo > 0.5 * wref) m2 = 0.5 * (lo + hi);
+      wref = hi - lo;
+      since = 0;
+    }
     m0 = m1; g0 = g1; m1 = m2;
   }
   return false;
 }
 
-// evaporation :145-203 (no property-table psat): mdot [kg/s] <= 0, Qd [W], ovr
+// evaporation :145-203 (psat_ext: the property table's psatExt): mdot [kg/s] <= 0, Qd [W], ovr
 // when the model's gas-side heat replaces the Nusselt one
 static inline void evaporation(double rhog, double Tg, double gamma, double Rg, double mug, double kg, double Tp,
                                double dp, double Re, int evap, int intf, const double* ep, double& mdot, double& Qd,
-                               bool& ovr, int* iters = nullptr, double seed = 0.0) {
+                               bool& ovr, int* iters = nullptr, double seed = 0.0, const double* psat_ext = nullptr) {
   mdot = 0.0;
   Qd = 0.0;
   ovr = false;
@@ -271,7 +279,7 @@ static inline void evaporation(double rhog, double Tg, double gamma, double Rg, 
   s.cpg = gamma * Rg / (gamma - 1.0);
   s.p = rhog * Rg * Tg;
   s.Mg = RU / Rg;
-  double psat = psat_cc(Tp, ep[EP_LVMV_RU], ep[EP_INVTB]);
+  double psat = psat_ext ? *psat_ext : psat_cc(Tp, ep[EP_LVMV_RU], ep[EP_INVTB]);
   double Xs = psat >= s.p ? 1.0 - XSCAP : fmin2(psat / s.p, 1.0 - XSCAP);
   double Ys = molar2mass(Xs, ep[EP_MV], s.Mg);
   if (Ys <= ep[EP_YINF]) return;

@@ -37,11 +37,75 @@ struct Mat {
   double rho, cs, lv, emiss;
   double ep[8];
   int evap, intf;
+  // property table (Config_Types_m.f90:273-291): ICE_PROP_NCOL columns over T = tmin..tmax
+  const double* tab = nullptr;
+  int tmin = 0, tmax = 0, flags = 0;
 };
 
 // NaN-ignoring min / max of two (Fortran MIN/MAX as gfortran compiles them)
 static inline double fmin2(double a, double b) { return (b < a || a != a) ? b : a; }
 static inline double fmax2(double a, double b) { return (b > a || a != a) ? b : a; }
+
+// ---------------------------------------------------------------- Lib_Properties.f90
+// tab(T) of one column, T an integer kelvin in tmin..tmax
+static inline double tab_at(const Mat& m, int col, int T) {
+  return m.tab[(size_t)col * (size_t)(m.tmax - m.tmin + 1) + (size_t)(T - m.tmin)];
+}
+// table_value :233-250
+static inline double table_value(const Mat& m, int col, double T) {
+  if (T != T) return T;
+  double Tc = std::min(std::max(T, (double)m.tmin), (double)m.tmax);
+  if (Tc >= (double)m.tmax) return tab_at(m, col, m.tmax);
+  int i = (int)Tc;
+  return tab_at(m, col, i) + (tab_at(m, col, i + 1) - tab_at(m, col, i)) * (Tc - (double)i);
+}
+// mat_rho :18-29 (rho holds rho_tab(T_min) when a table is set)
+static inline double mat_rho(const Mat& m, double T) {
+  return (m.flags & ICE_PROP_RHO_VARIES) ? table_value(m, ICE_PROP_RHO, T) : m.rho;
+}
+// mat_lv :97-106, mat_emiss :123-132
+static inline double mat_lv(const Mat& m, double T) {
+  return (m.flags & ICE_PROP_USE_LV) ? table_value(m, ICE_PROP_LV, T) : m.lv;
+}
+static inline double mat_emiss(const Mat& m, double T) {
+  return (m.flags & ICE_PROP_USE_EMISS) ? table_value(m, ICE_PROP_EMISS, T) : m.emiss;
+}
+// mat_e :163-184
+static inline double mat_e(const Mat& m, double T) {
+  const int lo = m.tmin, hi = m.tmax;
+  if (T != T) return T;
+  if (T < (double)lo) {
+    if (lo > 0) return tab_at(m, ICE_PROP_E, lo) * (T / (double)lo);
+    return (tab_at(m, ICE_PROP_E, lo + 1) - tab_at(m, ICE_PROP_E, lo)) * T;
+  }
+  if (T >= (double)hi)
+    return tab_at(m, ICE_PROP_E, hi) + (tab_at(m, ICE_PROP_E, hi) - tab_at(m, ICE_PROP_E, hi - 1)) * (T - (double)hi);
+  int i = (int)T;
+  return tab_at(m, ICE_PROP_E, i) + (tab_at(m, ICE_PROP_E, i + 1) - tab_at(m, ICE_PROP_E, i)) * (T - (double)i);
+}
+// mat_T_from_e :188-218
+static inline double mat_T_from_e(const Mat& m, double e) {
+  const int lo = m.tmin, hi = m.tmax;
+  if (e != e) return e;
+  if (e < tab_at(m, ICE_PROP_E, lo)) {
+    if (lo > 0) return (double)lo * (e / tab_at(m, ICE_PROP_E, lo));
+    return e / (tab_at(m, ICE_PROP_E, lo + 1) - tab_at(m, ICE_PROP_E, lo));
+  }
+  if (e >= tab_at(m, ICE_PROP_E, hi))
+    return (double)hi + (e - tab_at(m, ICE_PROP_E, hi)) / (tab_at(m, ICE_PROP_E, hi) - tab_at(m, ICE_PROP_E, hi - 1));
+  int i = lo, j = hi;
+  while (j - i > 1) {
+    int k = (i + j) / 2;
+    if (tab_at(m, ICE_PROP_E, k) <= e) i = k;
+    else j = k;
+  }
+  return (double)i + (e - tab_at(m, ICE_PROP_E, i)) / (tab_at(m, ICE_PROP_E, i + 1) - tab_at(m, ICE_PROP_E, i));
+}
+// mat_T_from_rhoe :224-229
+static inline double mat_T_from_rhoe(const Mat& m, double rhoe, double rho, double eps) {
+  return mat_T_from_e(m, rhoe / (rho + eps / (tab_at(m, ICE_PROP_E, m.tmin + 1) - tab_at(m, ICE_PROP_E, m.tmin))));
+}
+static inline bool cs_varies(const Mat& m) { return (m.flags & ICE_PROP_CS_VARIES) != 0; }
 
 // Finite_m.f90:15-23: NaN or infinity, by the exponent bits
 static inline bool not_finite(double x) {
@@ -75,12 +139,12 @@ static inline void p2c(int model, const double* q, double* c, const Mat& m) {
   double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
   if (model == MK) {
     c[0] = q[0]; c[1] = q[0] * q[1]; c[2] = q[0] * q[2]; c[3] = q[0] * q[3];
-    c[4] = q[0] * m.cs * q[4] + 0.5 * q[0] * n2;
+    c[4] = (cs_varies(m) ? q[0] * mat_e(m, q[4]) : q[0] * m.cs * q[4]) + 0.5 * q[0] * n2;
     c[5] = q[5];
   } else if (model == IG) {
     c[0] = q[0]; c[1] = q[0] * q[1]; c[2] = q[0] * q[2]; c[3] = q[0] * q[3];
     c[4] = q[0] * n2 + 3.0 * q[4];
-    c[5] = q[0] * m.cs * q[5] + 0.5 * c[4];
+    c[5] = (cs_varies(m) ? q[0] * mat_e(m, q[5]) : q[0] * m.cs * q[5]) + 0.5 * c[4];
     c[6] = q[6];
   } else {
     c[0] = q[0]; c[1] = q[0] * q[1]; c[2] = q[0] * q[2]; c[3] = q[0] * q[3];
@@ -90,7 +154,7 @@ static inline void p2c(int model, const double* q, double* c, const Mat& m) {
     c[7] = q[0] * q[2] * q[2] + q[7];
     c[8] = q[0] * q[2] * q[3] + q[8];
     c[9] = q[0] * q[3] * q[3] + q[9];
-    c[10] = q[0] * m.cs * q[10] + 0.5 * (c[4] + c[7] + c[9]);
+    c[10] = (cs_varies(m) ? q[0] * mat_e(m, q[10]) : q[0] * m.cs * q[10]) + 0.5 * (c[4] + c[7] + c[9]);
     c[11] = q[11];
   }
 }
@@ -102,7 +166,8 @@ static inline void c2p(int model, const double* c, double* q, const Mat& m) {
     q[0] = c[0];
     q[1] = c[1] / (q[0] + eps); q[2] = c[2] / (q[0] + eps); q[3] = c[3] / (q[0] + eps);
     double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-    q[4] = (c[4] - 0.5 * q[0] * n2) / (m.cs * q[0] + eps);
+    q[4] = cs_varies(m) ? mat_T_from_rhoe(m, c[4] - 0.5 * q[0] * n2, q[0], eps)
+                         : (c[4] - 0.5 * q[0] * n2) / (m.cs * q[0] + eps);
     q[5] = c[5];
     if (q[0] < eps) for (int v = 0; v < 6; ++v) q[v] = eps;
     if (q[4] < eps) q[4] = eps;
@@ -113,7 +178,7 @@ static inline void c2p(int model, const double* c, double* q, const Mat& m) {
     double n2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
     q[4] = 1.0 / 3.0 * (c[4] - q[0] * n2);
     if (q[0] <= RHO_EMPTY && q[4] > q[0] * THETA_CAP) q[4] = q[0] * THETA_CAP;
-    q[5] = (c[5] - 0.5 * c[4]) / (m.cs * q[0] + eps);
+    q[5] = cs_varies(m) ? mat_T_from_rhoe(m, c[5] - 0.5 * c[4], q[0], eps) : (c[5] - 0.5 * c[4]) / (m.cs * q[0] + eps);
     q[6] = c[6];
     if (q[0] < eps) for (int v = 0; v < 7; ++v) q[v] = eps;
     if (q[4] < eps) q[4] = eps;
@@ -128,7 +193,8 @@ static inline void c2p(int model, const double* c, double* q, const Mat& m) {
     q[7] = c[7] - q[0] * q[2] * q[2];
     q[8] = c[8] - q[0] * q[2] * q[3];
     q[9] = c[9] - q[0] * q[3] * q[3];
-    q[10] = (c[10] - 0.5 * (c[4] + c[7] + c[9])) / (m.cs * q[0] + eps);
+    q[10] = cs_varies(m) ? mat_T_from_rhoe(m, c[10] - 0.5 * (c[4] + c[7] + c[9]), q[0], eps)
+                          : (c[10] - 0.5 * (c[4] + c[7] + c[9])) / (m.cs * q[0] + eps);
     q[11] = c[11];
     if (q[0] <= RHO_EMPTY && q[4] + q[7] + q[9] > 3.0 * q[0] * THETA_CAP) {
       double f = 3.0 * q[0] * THETA_CAP / (q[4] + q[7] + q[9]);
@@ -180,7 +246,7 @@ static inline void flux_make(int model, const double* q, const double* nrm, doub
   if (model == MK) {
     f[0] = q[0] * un;
     f[1] = f[0] * q[1]; f[2] = f[0] * q[2]; f[3] = f[0] * q[3];
-    f[4] = f[0] * (0.5 * n2 + m.cs * q[4]);
+    f[4] = f[0] * (0.5 * n2 + (cs_varies(m) ? mat_e(m, q[4]) : m.cs * q[4]));
     f[5] = q[5] * un;
   } else if (model == IG) {
     f[0] = q[0] * un;
@@ -188,7 +254,7 @@ static inline void flux_make(int model, const double* q, const double* nrm, doub
     f[2] = f[0] * q[2] + q[4] * nrm[1];
     f[3] = f[0] * q[3] + q[4] * nrm[2];
     f[4] = f[0] * n2 + 5.0 * q[4] * un;
-    f[5] = f[0] * (0.5 * n2 + m.cs * q[5]) + 2.5 * q[4] * un;
+    f[5] = f[0] * (0.5 * n2 + (cs_varies(m) ? mat_e(m, q[5]) : m.cs * q[5])) + 2.5 * q[4] * un;
     f[6] = q[6] * un;
   } else {
     const double eps = EPS25;
@@ -205,7 +271,7 @@ static inline void flux_make(int model, const double* q, const double* nrm, doub
     f[7] = f[0] * q[2] * q[2] + un * q[7] + q[2] * p2n + q[2] * p2n;
     f[8] = f[0] * q[2] * q[3] + un * q[8] + q[2] * p3n + q[3] * p2n;
     f[9] = f[0] * q[3] * q[3] + un * q[9] + q[3] * p3n + q[3] * p3n;
-    f[10] = f[0] * (0.5 * n2 + 0.5 * (q[4] + q[7] + q[9]) / (q[0] + eps) + m.cs * q[10]) +
+    f[10] = f[0] * (0.5 * n2 + 0.5 * (q[4] + q[7] + q[9]) / (q[0] + eps) + (cs_varies(m) ? mat_e(m, q[10]) : m.cs * q[10])) +
             q[1] * p1n + q[2] * p2n + q[3] * p3n;
     f[11] = q[11] * un;
   }
@@ -220,7 +286,7 @@ static inline void source_make(int model, const double* q, const double* F, doub
     s[1] = -F[0] * q[1] + q[0] * F[1] / (F[5] + eps);
     s[2] = -F[0] * q[2] + q[0] * F[2] / (F[5] + eps);
     s[3] = -F[0] * q[3] + q[0] * F[3] / (F[5] + eps);
-    s[4] = -(F[0] * (0.5 * n2 + m.cs * q[4])) - F[0] * m.lv + F[4] +
+    s[4] = -(F[0] * (0.5 * n2 + (cs_varies(m) ? mat_e(m, q[4]) : m.cs * q[4]))) - F[0] * mat_lv(m, q[4]) + F[4] +
            q[0] * (F[1] * q[1] + F[2] * q[2] + F[3] * q[3]) / (F[5] + eps);
     s[5] = 0.0;
   } else if (model == IG) {
@@ -232,7 +298,7 @@ static inline void source_make(int model, const double* q, const double* F, doub
     s[4] = -(F[0] * (n2 + 3.0 * q[4] / q[0])) +
            2.0 * q[0] * (F[1] * q[1] + F[2] * q[2] + F[3] * q[3]) / (F[5] + eps) -
            6.0 * q[4] / (F[5] + eps);
-    s[5] = -(F[0] * m.cs * q[5]) - F[0] * m.lv + F[4] + 0.5 * s[4];
+    s[5] = -(cs_varies(m) ? F[0] * mat_e(m, q[5]) : F[0] * m.cs * q[5]) - F[0] * mat_lv(m, q[5]) + F[4] + 0.5 * s[4];
     s[6] = 0.0;
   } else {
     s[0] = -F[0];
@@ -251,7 +317,8 @@ static inline void source_make(int model, const double* q, const double* F, doub
            2.0 * q[8] / (F[5] + eps);
     s[9] = -(F[0] * (q[3] * q[3] + q[9] / q[0])) + q[0] * (F[3] * q[3] + F[3] * q[3]) / (F[5] + eps) -
            2.0 * q[9] / (F[5] + eps);
-    s[10] = -(F[0] * m.cs * q[10]) - F[0] * m.lv + F[4] + 0.5 * s[4] + 0.5 * s[7] + 0.5 * s[9];
+    s[10] = -(cs_varies(m) ? F[0] * mat_e(m, q[10]) : F[0] * m.cs * q[10]) - F[0] * mat_lv(m, q[10]) + F[4] + 0.5 * s[4] +
+             0.5 * s[7] + 0.5 * s[9];
     s[11] = 0.0;
   }
 }
@@ -283,14 +350,14 @@ static inline void riemann_saurel(const double* q1, const double* q4, const doub
     double uu = vn1 * n[0] + ut, vv = vn1 * n[1] + vt, ww = vn1 * n[2] + wt;
     f[0] = q1[0] * vn;
     f[1] = uu * f[0]; f[2] = vv * f[0]; f[3] = ww * f[0];
-    f[4] = f[0] * (0.5 * (uu * uu + vv * vv + ww * ww) + m.cs * q1[4]);
+    f[4] = f[0] * (0.5 * (uu * uu + vv * vv + ww * ww) + (cs_varies(m) ? mat_e(m, q1[4]) : m.cs * q1[4]));
     f[5] = q1[5] * vn;
   } else if (vn < 0.0) {
     double ut = q4[1] - vn4 * n[0], vt = q4[2] - vn4 * n[1], wt = q4[3] - vn4 * n[2];
     double uu = vn4 * n[0] + ut, vv = vn4 * n[1] + vt, ww = vn4 * n[2] + wt;
     f[0] = q4[0] * vn;
     f[1] = uu * f[0]; f[2] = vv * f[0]; f[3] = ww * f[0];
-    f[4] = f[0] * (0.5 * (uu * uu + vv * vv + ww * ww) + m.cs * q4[4]);
+    f[4] = f[0] * (0.5 * (uu * uu + vv * vv + ww * ww) + (cs_varies(m) ? mat_e(m, q4[4]) : m.cs * q4[4]));
     f[5] = q4[5] * vn;
   } else {
     for (int v = 0; v < 6; ++v) f[v] = 0.0;

@@ -21,6 +21,12 @@ int orc_step(orc_ctx*, int, ice_status_t*);
 int orc_get_source(orc_ctx*, int, int, double*, double*);
 int orc_residual_norm(orc_ctx*, double*, double*);
 int orc_inflow400_lines(orc_ctx*, ice_inflow400_line_t*, int, int32_t*, int64_t*);
+int orc_set_properties(orc_ctx*, int, int, int, const double*, int32_t);
+int orc_set_keep_source(orc_ctx*, int);
+int orc_restrict(orc_ctx*, orc_ctx*);
+int orc_prolong(orc_ctx*, orc_ctx*);
+int orc_set_res_threshold(orc_ctx*, double);
+int orc_res_stop(orc_ctx*, int32_t*, double*);
 int orc_inflow400_totals(orc_ctx*, ice_inflow400_total_t*, int, int32_t*);
 int orc_last_error(orc_ctx*, char*, int);
 void orc_destroy(orc_ctx*);
@@ -41,7 +47,11 @@ int ice_gpu_put_gas(ice_gpu_ctx* x, int b, const double* g, const double* R, con
 int ice_gpu_put_gas_yinf(ice_gpu_ctx* x, int b, const double* y) { return orc_put_gas_yinf(C(x), b, y); }
 int ice_gpu_put_tau(ice_gpu_ctx* x, int b, int p, const double* t) { return orc_put_tau(C(x), b, p, t); }
 int ice_gpu_step(ice_gpu_ctx* x, int n, ice_status_t* st) { return orc_step(C(x), n, st); }
-int ice_gpu_set_keep_source(ice_gpu_ctx* x, int on) { (void)x; (void)on; return 0; }
+int ice_gpu_restrict(ice_gpu_ctx* f, ice_gpu_ctx* c) { return orc_restrict(C(f), C(c)); }
+int ice_gpu_prolong(ice_gpu_ctx* c, ice_gpu_ctx* f) { return orc_prolong(C(c), C(f)); }
+int ice_gpu_set_keep_source(ice_gpu_ctx* x, int on) { return orc_set_keep_source(C(x), on); }
+int ice_gpu_set_res_threshold(ice_gpu_ctx* x, double t) { return orc_set_res_threshold(C(x), t); }
+int ice_gpu_res_stop(ice_gpu_ctx* x, int32_t* s, double* r) { return orc_res_stop(C(x), s, r); }
 int ice_gpu_get_source(ice_gpu_ctx* x, int b, int p, double* s, double* t) { return orc_get_source(C(x), b, p, s, t); }
 int ice_gpu_residual_norm(ice_gpu_ctx* x, double* t, double* f) { return orc_residual_norm(C(x), t, f); }
 int ice_gpu_timing(ice_gpu_ctx* x, double* s, double* k, int64_t* n) {
@@ -53,6 +63,9 @@ int ice_gpu_timing(ice_gpu_ctx* x, double* s, double* k, int64_t* n) {
 }
 int ice_gpu_set_tiles(ice_gpu_ctx* x, int a, int b, int c) { (void)x; (void)a; (void)b; (void)c; return 0; }
 int ice_gpu_set_kernel_timing(ice_gpu_ctx* x, int on) { (void)x; (void)on; return 0; }
+int ice_gpu_set_properties(ice_gpu_ctx* x, int p, int t0, int n, const double* c, int32_t f) {
+  return orc_set_properties(C(x), p, t0, n, c, f);
+}
 int ice_gpu_inflow400_lines(ice_gpu_ctx* x, ice_inflow400_line_t* o, int m, int32_t* n, int64_t* d) {
   return orc_inflow400_lines(C(x), o, m, n, d);
 }

@@ -17,6 +17,8 @@ module ice_ref_shim
   use ICE_Lib_Heat,           only: heat
   use ICE_Lib_RK,             only: RK_stage
   use ICE_Lib_Evaporation,    only: evaporation, blowingFactor, nep
+  use ICE_Lib_Properties,     only: mat_rho, mat_e, mat_T_from_e, mat_T_from_rhoe, mat_lv, mat_cpv, mat_emiss, &
+                                    mat_psat
   implicit none
   type(condensed_phase_t), save :: mat
   character(len=16), parameter :: lim_words(0:10) = [character(len=16) :: 'IORD', 'minmod', 'vanalbada', &
@@ -40,6 +42,58 @@ contains
     mat%rho_al = rho; mat%cs_al = cs; mat%lv_al = lv; mat%emiss = emiss
     mat%use_table = .false.; mat%cs_varies = .false.; mat%rho_varies = .false.
   end subroutine ref_setup
+
+  ! a property table on T = t_min .. t_min+n-1 in ice_gpu_set_properties' layout: columns
+  ! density, cp, e, Psat, Lv, Cpv, emissivity; flags 1 rho varies, 2 cs varies, 4 Psat,
+  ! 8 Lv, 16 Cpv, 32 emissivity
+  subroutine ref_setup_table(t_min, n, cols, flags) bind(C, name='ref_setup_table')
+    integer(c_int), value :: t_min, n, flags
+    real(c_double), intent(in) :: cols(n, 7)
+    integer :: hi
+    hi = t_min + n - 1
+    if (allocated(mat%rho_tab)) deallocate(mat%rho_tab, mat%cs_tab, mat%e_tab, mat%psat_tab, mat%lv_tab, &
+                                           mat%cpv_tab, mat%emiss_tab)
+    allocate(mat%rho_tab(t_min:hi), source=cols(:, 1))
+    allocate(mat%cs_tab(t_min:hi), source=cols(:, 2))
+    allocate(mat%e_tab(t_min:hi), source=cols(:, 3))
+    allocate(mat%psat_tab(t_min:hi), source=cols(:, 4))
+    allocate(mat%lv_tab(t_min:hi), source=cols(:, 5))
+    allocate(mat%cpv_tab(t_min:hi), source=cols(:, 6))
+    allocate(mat%emiss_tab(t_min:hi), source=cols(:, 7))
+    mat%T_min = t_min; mat%T_max = hi
+    mat%use_table  = .true.
+    mat%rho_varies = iand(flags, 1) /= 0
+    mat%cs_varies  = iand(flags, 2) /= 0
+    mat%use_psat   = iand(flags, 4) /= 0
+    mat%use_lv     = iand(flags, 8) /= 0
+    mat%use_cpv    = iand(flags, 16) /= 0
+    mat%use_emiss  = iand(flags, 32) /= 0
+  end subroutine ref_setup_table
+
+  ! one property of the table at T: 1 density, 4 Psat, 5 Lv, 6 Cpv, 7 emissivity, as the
+  ! reference's mat_* functions return them; 3 the energy mat_e
+  real(c_double) function ref_mat(col, T) bind(C, name='ref_mat')
+    integer(c_int), value :: col
+    real(c_double), value :: T
+    select case (col)
+    case (1); ref_mat = mat_rho(mat, T)
+    case (3); ref_mat = mat_e(mat, T)
+    case (4); ref_mat = mat_psat(mat, T)
+    case (5); ref_mat = mat_lv(mat, T)
+    case (6); ref_mat = mat_cpv(mat, T)
+    case default; ref_mat = mat_emiss(mat, T)
+    end select
+  end function ref_mat
+
+  real(c_double) function ref_mat_T_from_e(e) bind(C, name='ref_mat_T_from_e')
+    real(c_double), value :: e
+    ref_mat_T_from_e = mat_T_from_e(mat, e)
+  end function ref_mat_T_from_e
+
+  real(c_double) function ref_mat_T_from_rhoe(rhoe, rho, eps) bind(C, name='ref_mat_T_from_rhoe')
+    real(c_double), value :: rhoe, rho, eps
+    ref_mat_T_from_rhoe = mat_T_from_rhoe(mat, rhoe, rho, eps)
+  end function ref_mat_T_from_rhoe
 
   subroutine ref_p2c(nc, prim, cons) bind(C, name='ref_p2c')
     integer(c_int), value :: nc
@@ -148,6 +202,18 @@ contains
     call evaporation(rhog, Tg, gam, Rg, mug, kg, Tp, dp, Re, evap, intf, ep, mdot, qdot, o)
     ovr = merge(1, 0, o)
   end subroutine ref_evaporation
+
+  subroutine ref_evaporation_psat(rhog, Tg, gam, Rg, mug, kg, Tp, dp, Re, evap, intf, ep, psat, mdot, qdot, ovr) &
+      bind(C, name='ref_evaporation_psat')
+    real(c_double), value :: rhog, Tg, gam, Rg, mug, kg, Tp, dp, Re, psat
+    integer(c_int), value :: evap, intf
+    real(c_double), intent(in) :: ep(nep)
+    real(c_double), intent(out) :: mdot, qdot
+    integer(c_int), intent(out) :: ovr
+    logical :: o
+    call evaporation(rhog, Tg, gam, Rg, mug, kg, Tp, dp, Re, evap, intf, ep, mdot, qdot, o, psatExt=psat)
+    ovr = merge(1, 0, o)
+  end subroutine ref_evaporation_psat
 
   real(c_double) function ref_blowing(gam, Rg, mug, kg, rhop, dp, mp, mdot) bind(C, name='ref_blowing')
     real(c_double), value :: gam, Rg, mug, kg, rhop, dp, mp, mdot
