@@ -62,7 +62,12 @@ module ice_gpu_m
   public :: ice_gpu_create, ice_gpu_set_geometry, ice_gpu_set_boundaries, ice_gpu_put_state, ice_gpu_get_state, &
             ice_gpu_put_gas, ice_gpu_put_tau, ice_gpu_step, ice_gpu_residual_norm, ice_gpu_timing, &
             ice_gpu_last_error, ice_gpu_destroy, ice_gpu_error_text, ice_gpu_autotune, ice_gpu_inflow400_lines, &
-            ice_gpu_inflow400_totals, ice_gpu_set_properties
+            ice_gpu_inflow400_totals, ice_gpu_set_properties, ice_gpu_get_bc_maps, ice_gpu_get_source, &
+            ice_gpu_put_gas_yinf, ice_gpu_set_keep_source, ice_gpu_set_res_threshold, ice_gpu_res_stop, &
+            ice_gpu_restrict, ice_gpu_prolong, ice_gpu_set_tiles, ice_gpu_set_dispatch, ice_gpu_set_kernel_timing, &
+            ice_gpu_mlp_grad, ice_gpu_mlp_forward, ice_gpu_surrogate_train, ice_gpu_surrogate_retrain_every, &
+            ice_gpu_surrogate_params, ice_gpu_surrogate_set_params, ice_gpu_set_lk_stats, ice_gpu_lk_stats
+  integer, parameter, public :: ICE_MLP_PARAMS = 6288
 
   interface
     integer(c_int) function ice_gpu_create(device, cfg, fams, nb, dims, ctx) bind(C, name='ice_gpu_create')
@@ -157,6 +162,114 @@ module ice_gpu_m
       type(ice_inflow400_total_t) :: totals(*)
       integer(c_int), value :: max
       integer(c_int32_t) :: n
+    end function
+    integer(c_int) function ice_gpu_get_bc_maps(ctx, partner, owner, cf, p, ncell, first, rec, cap) &
+        bind(C, name='ice_gpu_get_bc_maps')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: partner(*), owner(*), cf(*), ncell, first(*), rec(*)
+      integer(c_int), value :: p, cap
+    end function
+    integer(c_int) function ice_gpu_get_source(ctx, b, p, source, tau) bind(C, name='ice_gpu_get_source')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: b, p
+      real(c_double) :: source(*), tau(*)
+    end function
+    integer(c_int) function ice_gpu_put_gas_yinf(ctx, b, yinf) bind(C, name='ice_gpu_put_gas_yinf')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: b
+      real(c_double) :: yinf(*)
+    end function
+    integer(c_int) function ice_gpu_set_keep_source(ctx, on) bind(C, name='ice_gpu_set_keep_source')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function ice_gpu_set_res_threshold(ctx, threshold) bind(C, name='ice_gpu_set_res_threshold')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: threshold
+    end function
+    integer(c_int) function ice_gpu_res_stop(ctx, stopped, reference) bind(C, name='ice_gpu_res_stop')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: stopped
+      real(c_double) :: reference
+    end function
+    integer(c_int) function ice_gpu_restrict(fine, coarse) bind(C, name='ice_gpu_restrict')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: fine, coarse
+    end function
+    integer(c_int) function ice_gpu_prolong(coarse, fine) bind(C, name='ice_gpu_prolong')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: coarse, fine
+    end function
+    integer(c_int) function ice_gpu_set_tiles(ctx, tx, ty, tz) bind(C, name='ice_gpu_set_tiles')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: tx, ty, tz
+    end function
+    integer(c_int) function ice_gpu_set_dispatch(ctx, dispatch) bind(C, name='ice_gpu_set_dispatch')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: dispatch
+    end function
+    integer(c_int) function ice_gpu_set_kernel_timing(ctx, on) bind(C, name='ice_gpu_set_kernel_timing')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function ice_gpu_mlp_grad(ctx, n, x, y, params, grad, loss) bind(C, name='ice_gpu_mlp_grad')
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+      real(c_float) :: x(*), y(*), params(*), grad(*), loss
+    end function
+    integer(c_int) function ice_gpu_mlp_forward(ctx, n, x, params, out) bind(C, name='ice_gpu_mlp_forward')
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+      real(c_float) :: x(*), params(*), out(*)
+    end function
+    integer(c_int) function ice_gpu_surrogate_train(ctx, p, epochs, batch, lr, seed, report) &
+        bind(C, name='ice_gpu_surrogate_train')
+      import :: c_int, c_double, c_float, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: p, epochs, batch
+      real(c_double), value :: lr
+      integer(c_int64_t), value :: seed
+      real(c_float) :: report(6)
+    end function
+    integer(c_int) function ice_gpu_surrogate_retrain_every(ctx, every, epochs) &
+        bind(C, name='ice_gpu_surrogate_retrain_every')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: every, epochs
+    end function
+    integer(c_int) function ice_gpu_surrogate_params(ctx, p, params) bind(C, name='ice_gpu_surrogate_params')
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: p
+      real(c_float) :: params(*)
+    end function
+    integer(c_int) function ice_gpu_surrogate_set_params(ctx, p, params) bind(C, name='ice_gpu_surrogate_set_params')
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: p
+      real(c_float) :: params(*)
+    end function
+    integer(c_int) function ice_gpu_set_lk_stats(ctx, on) bind(C, name='ice_gpu_set_lk_stats')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function ice_gpu_lk_stats(ctx, p, counts) bind(C, name='ice_gpu_lk_stats')
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: p
+      integer(c_int64_t) :: counts(4)
     end function
     integer(c_int) function ice_gpu_last_error(ctx, buf, len) bind(C, name='ice_gpu_last_error')
       import :: c_int, c_char, c_ptr

@@ -47,3 +47,11 @@ def test_load_without_gpu():
     """The library loads (dlopen) on a host without a GPU."""
     lib = native.load()
     assert hasattr(lib, "ice_gpu_step")
+
+
+def test_fortran_module_binds_every_symbol():
+    """ice_gpu_m.f90 has a bind(C) interface for every function the header declares"""
+    src = open(os.path.join(ROOT, "ice_amd", "fortran", "ice_gpu_m.f90")).read()
+    bound = set(re.findall(r"name='(ice_gpu_[a-z0-9_]+)'", src))
+    missing = [s for s in declared() if s not in bound]
+    assert not missing, missing
