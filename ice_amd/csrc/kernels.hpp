@@ -206,7 +206,12 @@ ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double*
     Qconv = Qconv * blowing_factor_d(gam, R, mu, gk, rho_mat, 2.0 * Rp, 4.0 / 3.0 * kPi * (Rp * Rp * Rp) * rho_mat, mdot);
   F[0] = -mdot * q[NC - 1];
   F[4] = Qconv * q[NC - 1];
-  F[4] = F[4] + mat_emiss(mt, Ts) * kSigmaSB * 4.0 * kPi * Rp * Rp * q[NC - 1] * (pow(gp[4], 4.0) - pow(Ts, 4.0));
+  // grey-body radiation; with a zero emissivity the term is 0 times a finite
+  // product (the temperatures of a state that passed check_prim are finite), so
+  // its two powers are not evaluated
+  const double emiss = mat_emiss(mt, Ts);
+  if (emiss != 0.0)
+    F[4] = F[4] + emiss * kSigmaSB * 4.0 * kPi * Rp * Rp * q[NC - 1] * (pow(gp[4], 4.0) - pow(Ts, 4.0));
   if (Cd == 0.0) tau = __longlong_as_double(0x7FF0000000000000ll);
   else tau = 8.0 * rho_mat * Rp / (3.0 * gp[0] * Cd * slip + kTauToll);
   F[5] = tau;

@@ -25,19 +25,25 @@ module ice_host_m
 
   !> The registry: every key the path reads, with its section and default
   !> (Register_Numerics.f90:28-80, Register_Physics.f90, Read_Ini.f90)
-  integer, parameter :: NKEY = 21
+  !> ([ICE-Multigrid] levels and level<m>-iter, Register_Numerics.f90:90-115: up to
+  !> MAXLEV levels here)
+  integer, parameter, public :: MAXLEV = 4
+  integer, parameter :: NKEY = 27
   character(len=24), parameter :: key_sec(NKEY) = [character(len=24) :: &
     'ICE-Parameters', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', &
     'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', &
     'ICE-Numerics', 'ICE-Physics', 'ICE-Physics', 'ICE-IO', 'ICE-IO', 'ICE-IO', 'ICE-Device', 'ICE-Family', &
-    'ICE-Device']
+    'ICE-Device', 'ICE-Parameters', 'ICE-Multigrid', 'ICE-Multigrid', 'ICE-Multigrid', 'ICE-Multigrid', &
+    'ICE-Multigrid']
   character(len=24), parameter :: key_name(NKEY) = [character(len=24) :: &
     'iter-threshold', 'time-scheme', 'cfl', 'dt-max', 'tau-factor', 'cfl-rise-threshold', 'time-accurate', &
     'irs', 'irs-beta', 'space-reconstruction', 'flux-limiter', 'shock-detector', 'riemann-solver', &
-    'drag', 'heat-transfer', 'ic-format', 'sol-format', 'shell-diter', 'mode', 'closure', 'autotune']
+    'drag', 'heat-transfer', 'ic-format', 'sol-format', 'shell-diter', 'mode', 'closure', 'autotune', &
+    'res-threshold', 'levels', 'level1-iter', 'level2-iter', 'level3-iter', 'level4-iter']
   character(len=24), parameter :: key_default(NKEY) = [character(len=24) :: &
     '1000000000', 'RK2', '0.5', '1e-4', '1.0', '0', '.true.', '.false.', '0.5', 'first-order', 'none', 'none', &
-    '', 'NoDrag', 'NoHeat', 'tecplot ascii', 'tecplot ascii', '10', 'plain', 'IG', '.false.']
+    '', 'NoDrag', 'NoHeat', 'tecplot ascii', 'tecplot ascii', '10', 'plain', 'IG', '.false.', &
+    '0.0', '1', '1000000000', '1000000000', '1000000000', '1000000000']
 
   type, public :: registry_t
     character(len=64) :: value(NKEY)
@@ -46,7 +52,7 @@ module ice_host_m
   end type
 
   public :: read_registry, reg_str, reg_real, reg_int, reg_bool, config_from_registry
-  public :: read_ic, read_bc, read_material, block_metrics, write_solution, refuse
+  public :: read_ic, read_bc, read_material, block_metrics, write_solution, refuse, coarse_blocks
 
 contains
 
@@ -439,6 +445,29 @@ contains
       b = index(line(a:), '"')
       if (b == 0) exit
       a = a + b - 1
+    end do
+  end subroutine
+
+  !> The next coarser level's blocks (Setup_Multigrid, Mod_Multigrid.f90:37-70, and
+  !> Check_Multigrid, Lib_Multigrid.f90:12-35): half the cells in each direction, a
+  !> flat block keeping nk = 1, its nodes every other node of the finer level
+  !> (Coarse_Grid, :194-213), with their metrics
+  subroutine coarse_blocks(fine, coarse)
+    type(host_block_t), intent(in) :: fine(:)
+    type(host_block_t), allocatable, intent(out) :: coarse(:)
+    integer :: b, step3
+    allocate(coarse(size(fine)))
+    do b = 1, size(fine)
+      if (mod(fine(b)%dim(1), 2) /= 0 .or. mod(fine(b)%dim(2), 2) /= 0 .or. &
+          (fine(b)%dim(3) /= 1 .and. mod(fine(b)%dim(3), 2) /= 0)) &
+        call refuse('Check_Multigrid', 'block '//itoa(b)//' not divisible by 2^(MGL-1)')
+      coarse(b)%dim(1:2) = fine(b)%dim(1:2) / 2
+      coarse(b)%dim(3) = max(1, fine(b)%dim(3) / 2)
+      step3 = 2
+      if (fine(b)%dim(3) == 1) step3 = 1
+      allocate(coarse(b)%node(3, 0:coarse(b)%dim(1), 0:coarse(b)%dim(2), 0:coarse(b)%dim(3)))
+      coarse(b)%node = fine(b)%node(:, 0:fine(b)%dim(1):2, 0:fine(b)%dim(2):2, 0:fine(b)%dim(3):step3)
+      call block_metrics(coarse(b))
     end do
   end subroutine
 

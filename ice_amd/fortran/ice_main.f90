@@ -11,8 +11,14 @@ program ice_driver
   use ice_gpu_m
   use ice_host_m
   implicit none
-  character(len=1024) :: dir, arg
+  !> one grid level: its blocks and its context (Setup_Multigrid: a domain per level)
+  type :: level_t
+    type(host_block_t), allocatable :: b(:)
+    type(c_ptr) :: ctx = c_null_ptr
+  end type
+  character(len=1024) :: dir, arg, bcpath
   type(registry_t) :: reg
+  type(level_t) :: lev(MAXLEV)
   type(host_block_t), allocatable :: blocks(:)
   type(ice_config_t) :: cfg
   type(ice_family_t), allocatable :: fams(:)
@@ -20,13 +26,14 @@ program ice_driver
   integer(c_int32_t), allocatable :: reci(:,:), donor(:,:), dims(:)
   real(R8), allocatable :: recd(:,:), weight(:), fam(:), prop_cols(:,:)
   integer :: t_min, prop_flags
-  real(R8) :: rho_mat, cs, total(5), step_ms, kernel_ms, tile_ms(16)
-  integer(c_int32_t) :: tile(4)
+  real(R8) :: rho_mat, cs, total(5), step_ms, kernel_ms, tile_ms(16), res_ref
+  integer(c_int32_t) :: tile(4), stopped
   integer(c_int64_t) :: launches
   type(c_ptr) :: ctx
-  integer :: device, nb, b, nc, ndir, iters, it, n, shell, rc, ndonor
+  integer :: device, nb, b, nc, ndir, iters, it, n, shell, rc, ndonor, mgl, m, total_iters
   integer(I8) :: cells, t0, t1, rate
   character(len=96) :: names
+  character(len=16) :: key
 
   call get_command_argument(1, dir)
   if (len_trim(dir) == 0) call refuse('ice_driver', 'usage: ice_driver <case directory> [device]')
@@ -41,6 +48,8 @@ program ice_driver
   case ('IG'); nc = 7;  names = 'rho_p1 u_p1 v_p1 w_p1 P_p1 T_p1 n_p1'
   case default; nc = 12; names = 'rho_p1 u_p1 v_p1 w_p1 P11_p1 P12_p1 P13_p1 P22_p1 P23_p1 P33_p1 T_p1 n_p1'
   end select
+  mgl = reg_int(reg, 'levels')
+  if (mgl < 1 .or. mgl > MAXLEV) call refuse('Register_Multigrid_Levels', 'levels must be 1 to 4 here')
   call read_ic(trim(dir)//'/INPUT/part-ic.tec', nc, blocks)
   call read_material(trim(dir)//'/INPUT/part-properties.dat', rho_mat, cs, t_min, prop_cols, prop_flags)
   nb = size(blocks)
@@ -49,74 +58,132 @@ program ice_driver
     if (any([(blocks(b)%dim(n) > 1, b = 1, nb)])) ndir = ndir + 1
   end do
   call config_from_registry(reg, ndir, rho_mat, cs, cfg, fams)
-  call read_bc(trim(dir)//'/INPUT/part-bc.txt', blocks, cfg%ngroups, reci, recd, donor, weight)
-
-  allocate(dims(3 * nb))
-  cells = 0
   do b = 1, nb
-    dims(3 * b - 2:3 * b) = blocks(b)%dim
-    cells = cells + product(int(blocks(b)%dim, I8))
     call block_metrics(blocks(b))
   end do
-
-  call check(ice_gpu_create(device, cfg, fams, nb, dims, ctx), 'create')
-  call check(ice_gpu_set_properties(ctx, 1, t_min, size(prop_cols, 1), prop_cols, int(prop_flags, c_int32_t)), &
-             'set_properties')
-  do b = 1, nb
-    call check(ice_gpu_set_geometry(ctx, b, blocks(b)%vol, blocks(b)%dl, blocks(b)%m, blocks(b)%fi, &
-                                    blocks(b)%fj, blocks(b)%fk), 'set_geometry')
-  end do
-  ndonor = 0
-  if (any(reci(R_TYPE, :) == 102)) ndonor = size(weight)
-  call check(ice_gpu_set_boundaries(ctx, size(reci, 2), reci, recd, ndonor, donor, weight), 'set_boundaries')
-  do b = 1, nb
-    call check(ice_gpu_put_state(ctx, b, 1, blocks(b)%prim), 'put_state')
+  call move_alloc(blocks, lev(1)%b)
+  do m = 2, mgl
+    call coarse_blocks(lev(m - 1)%b, lev(m)%b)
   end do
 
-  if (reg_bool(reg, 'autotune')) then
-    call check(ice_gpu_autotune(ctx, 2, tile, tile_ms), 'autotune')
-    write(output_unit, '(A,3(1X,I0),A,I0)') ' autotuned stage tile:', tile(1:3), ', dispatch ', tile(4)
-  end if
-  iters = reg_int(reg, 'iter-threshold')
+  ! a context per level: geometry, boundary table (part-bc.txt, part-bc<m>.txt), material
+  allocate(dims(3 * nb))
+  do m = 1, mgl
+    do b = 1, nb
+      dims(3 * b - 2:3 * b) = lev(m)%b(b)%dim
+    end do
+    bcpath = trim(dir)//'/INPUT/part-bc.txt'
+    if (m > 1) write(bcpath, '(A,I0,A)') trim(dir)//'/INPUT/part-bc', m, '.txt'
+    call read_bc(trim(bcpath), lev(m)%b, cfg%ngroups, reci, recd, donor, weight)
+    ctx = c_null_ptr
+    call check(ice_gpu_create(device, cfg, fams, nb, dims, ctx), 'create')
+    lev(m)%ctx = ctx
+    call check(ice_gpu_set_properties(ctx, 1, t_min, size(prop_cols, 1), prop_cols, int(prop_flags, c_int32_t)), &
+               'set_properties')
+    do b = 1, nb
+      call check(ice_gpu_set_geometry(ctx, b, lev(m)%b(b)%vol, lev(m)%b(b)%dl, lev(m)%b(b)%m, lev(m)%b(b)%fi, &
+                                      lev(m)%b(b)%fj, lev(m)%b(b)%fk), 'set_geometry')
+    end do
+    ndonor = 0
+    if (any(reci(R_TYPE, :) == 102)) ndonor = size(weight)
+    call check(ice_gpu_set_boundaries(ctx, size(reci, 2), reci, recd, ndonor, donor, weight), 'set_boundaries')
+    if (m == 1) then
+      do b = 1, nb
+        call check(ice_gpu_put_state(ctx, b, 1, lev(1)%b(b)%prim), 'put_state')
+      end do
+      ! res-threshold ends steady runs, on the finest level (Mod_Explicit.f90:196-200)
+      call check(ice_gpu_set_res_threshold(ctx, reg_real(reg, 'res-threshold')), 'set_res_threshold')
+    else
+      ! the initial solution of each coarser level (Mod_Multigrid.f90:76)
+      ctx = lev(m - 1)%ctx
+      call check(ice_gpu_restrict(lev(m - 1)%ctx, lev(m)%ctx), 'restrict')
+    end if
+    if (reg_bool(reg, 'autotune')) then
+      ctx = lev(m)%ctx
+      call check(ice_gpu_autotune(ctx, 2, tile, tile_ms), 'autotune')
+      write(output_unit, '(A,I0,A,3(1X,I0),A,I0)') ' level ', m, ': autotuned stage tile:', tile(1:3), &
+        ', dispatch ', tile(4)
+    end if
+  end do
+
   shell = max(1, reg_int(reg, 'shell-diter'))
   allocate(fam(5 * cfg%ngroups))
-  write(output_unit, '(A,I0,A,I0,A,I0,A)') ' ICE device path: ', nb, ' blocks, ', cells, ' cells, ', iters, ' iterations'
-  call system_clock(t0, rate)
-  it = 0
-  do while (it < iters)
-    n = min(shell, iters - it)
-    rc = ice_gpu_step(ctx, n, st)
-    if (st%bad_recon /= 0) call refuse('state_reconstruction', 'unphysical state at first order')
-    if (st%bad_state /= 0) then
-      write(output_unit, '(A,5(1X,I0))') ' [ERROR] [ICE::state_update] non-physical state at block, i, j, k, family', &
-        st%bad_block, st%bad_i, st%bad_j, st%bad_k, st%bad_p
-      error stop 1
-    end if
-    call check(rc, 'step')
-    call print_inflow400_lines()
-    it = it + n
-    call check(ice_gpu_residual_norm(ctx, total, fam), 'residual_norm')
-    write(output_unit, '(A,I8,A,5ES13.5)') ' iter', st%iter, '  res', total
+  cells = 0
+  do b = 1, nb
+    cells = cells + product(int(lev(1)%b(b)%dim, I8))
   end do
+  write(output_unit, '(A,I0,A,I0,A,I0,A)') ' ICE device path: ', nb, ' blocks, ', cells, ' cells, ', mgl, ' levels'
+  call system_clock(t0, rate)
+  total_iters = 0
+  ! the levels, coarsest first, each for its level<m>-iter iterations, then prolongation
+  ! onto the next finer one (Wrap_Solve.f90:28-39)
+  do m = mgl, 1, -1
+    ctx = lev(m)%ctx
+    write(key, '(A,I0,A)') 'level', m, '-iter'
+    iters = reg_int(reg, trim(key))
+    if (m == 1) iters = min(iters, reg_int(reg, 'iter-threshold'))
+    it = 0
+    do while (it < iters)
+      n = min(shell, iters - it)
+      rc = ice_gpu_step(ctx, n, st)
+      if (st%bad_recon /= 0) call refuse('state_reconstruction', 'unphysical state at first order')
+      if (st%bad_state /= 0) then
+        write(output_unit, '(A,5(1X,I0))') ' [ERROR] [ICE::state_update] non-physical state at block, i, j, k, family', &
+          st%bad_block, st%bad_i, st%bad_j, st%bad_k, st%bad_p
+        error stop 1
+      end if
+      call check(rc, 'step')
+      call print_inflow400_lines(m)
+      call check(ice_gpu_res_stop(ctx, stopped, res_ref), 'res_stop')
+      if (stopped /= 0) then
+        n = st%iter - (total_iters + it)
+        it = it + n
+        call check(ice_gpu_residual_norm(ctx, total, fam), 'residual_norm')
+        write(output_unit, '(A,I8,A,5ES13.5)') ' iter', st%iter, '  res', total
+        write(output_unit, '(A,ES12.4,A)') ' converged: density residual below res-threshold x ', res_ref, &
+          ' (the first non-zero one)'
+        exit
+      end if
+      it = it + n
+      call check(ice_gpu_residual_norm(ctx, total, fam), 'residual_norm')
+      if (mgl > 1) then
+        write(output_unit, '(A,I0,A,I8,A,5ES13.5)') ' level ', m, ' iter', st%iter, '  res', total
+      else
+        write(output_unit, '(A,I8,A,5ES13.5)') ' iter', st%iter, '  res', total
+      end if
+    end do
+    total_iters = total_iters + it
+    if (m > 1) then
+      ctx = lev(m - 1)%ctx
+      call check(ice_gpu_prolong(lev(m)%ctx, lev(m - 1)%ctx), 'prolong')
+    end if
+  end do
+  ctx = lev(1)%ctx
   call system_clock(t1)
   call check(ice_gpu_timing(ctx, step_ms, kernel_ms, launches), 'timing')
-  write(output_unit, '(A,F10.3,A,F12.2,A)') ' wall ', real(t1 - t0, R8) / rate, ' s, ', &
-    real(cells * cfg%ngroups * cfg%nrk, R8) * iters / (real(t1 - t0, R8) / rate) / 1e6_R8, ' Mcell-stage/s'
+  write(output_unit, '(A,F10.3,A)') ' wall ', real(t1 - t0, R8) / rate, ' s'
   if (st%n_floored > 0) write(output_unit, '(A,I0)') ' [WARNING] floored densities: ', st%n_floored
 
-  call print_inflow400_totals()
+  do m = mgl, 1, -1
+    ctx = lev(m)%ctx
+    call print_inflow400_totals(m)
+  end do
 
+  ctx = lev(1)%ctx
   do b = 1, nb
-    call check(ice_gpu_get_state(ctx, b, 1, blocks(b)%prim), 'get_state')
+    call check(ice_gpu_get_state(ctx, b, 1, lev(1)%b(b)%prim), 'get_state')
   end do
   call execute_command_line('mkdir -p '//trim(dir)//'/OUTPUT')
-  call write_solution(trim(dir)//'/OUTPUT/part-field.tec', names, blocks, st%iter)
-  call ice_gpu_destroy(ctx)
+  call write_solution(trim(dir)//'/OUTPUT/part-field.tec', names, lev(1)%b, st%iter)
+  do m = 1, mgl
+    call ice_gpu_destroy(lev(m)%ctx)
+  end do
 
 contains
 
   !> The 400-inflow lines of the steps just run, in ICE's text (Mod_Inflow400.f90:118-123)
-  subroutine print_inflow400_lines()
+  subroutine print_inflow400_lines(level)
+    integer, intent(in) :: level
     type(ice_inflow400_line_t), save :: lines(ICE_INFLOW400_LOG)
     integer(c_int32_t) :: nl
     integer(c_int64_t) :: dropped
@@ -127,7 +194,7 @@ contains
       which = ', faces: '
       if (lines(m)%relative /= 0) which = ', faces above 10 times its last line: '
       write(output_unit, '(A,I0,A,I0,A)') ' 400 boundary of '//trim(boundary_name(lines(m)%block, lines(m)%face, &
-        lines(m)%family))//': particles entering at iteration ', lines(m)%iter, trim(which)//' ', lines(m)%faces, &
+        lines(m)%family, level))//': particles entering at iteration ', lines(m)%iter, trim(which)//' ', lines(m)%faces, &
         ', largest '//trim(measure_text(lines(m)%family, lines(m)%largest))
     end do
     if (dropped > 0) write(output_unit, '(A,I0,A)') ' [WARNING] ', dropped, ' lines of the 400 report were dropped'
@@ -135,7 +202,8 @@ contains
   end subroutine
 
   !> The boundaries through which particles entered over the run (Mod_Inflow400.f90:156-169)
-  subroutine print_inflow400_totals()
+  subroutine print_inflow400_totals(level)
+    integer, intent(in) :: level
     type(ice_inflow400_total_t), allocatable :: t(:)
     integer(c_int32_t) :: nt
     integer :: m
@@ -143,17 +211,19 @@ contains
     call check(ice_gpu_inflow400_totals(ctx, t, size(t), nt), 'inflow400_totals')
     do m = 1, nt
       write(output_unit, '(A,I0,A,I0,A,I0,A)') ' 400 boundary of '//trim(boundary_name(t(m)%block, t(m)%face, &
-        t(m)%family))//': particles entered in ', t(m)%stages, ' stages, first at iteration ', t(m)%first_iter, &
+        t(m)%family, level))//': particles entered in ', t(m)%stages, ' stages, first at iteration ', t(m)%first_iter, &
         ' (faces: ', t(m)%first_faces, ', largest '//trim(measure_text(t(m)%family, t(m)%first_largest))// &
         '), largest over the run: '//trim(number_text(t(m)%family, t(m)%largest))
     end do
   end subroutine
 
-  function boundary_name(b, f, p) result(s)
+  function boundary_name(b, f, p, level) result(s)
     integer(c_int32_t), intent(in) :: b, f, p
+    integer, intent(in) :: level
     character(len=80) :: s
     write(s, '(A,I0,A,I0)') 'block ', b, ', face ', f
     if (cfg%ngroups > 1) write(s, '(A,I0)') trim(s)//', family ', p
+    if (mgl > 1) write(s, '(A,I0)') trim(s)//', grid level ', level
   end function
 
   function measure_text(p, x) result(s)

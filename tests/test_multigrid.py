@@ -117,3 +117,57 @@ def test_gpu_sequence_coarse_then_fine(gpu, oracle, mode):
         rf.step(5)
         runs.append(rf)
     assert compare_states(runs[0], runs[1], fine) <= 1e-12
+
+
+def write_box_inputs(d, fine, levels, iters):
+    """A case_box level set in ICE's own input files: the finest level's IC and
+    boundary table, part-bc<m>.txt for the coarser ones, [ICE-Multigrid]."""
+    import os
+
+    from ice_amd import formats
+    os.makedirs(os.path.join(d, "INPUT"), exist_ok=True)
+    ini = {"ICE-Parameters": {"iter-threshold": str(iters[0])},
+           "ICE-Numerics": {"time-scheme": "RK2", "cfl": "0.5", "time-accurate": ".false.",
+                            "space-reconstruction": "MUSCL", "flux-limiter": "vanleer"},
+           "ICE-Family1": {"closure": "IG"},
+           "ICE-IO": {"shell-diter": "1000"},
+           "ICE-Multigrid": {"levels": str(levels), **{f"level{m + 1}-iter": str(n) for m, n in enumerate(iters)}}}
+    formats.write_ini(os.path.join(d, "input.ini"), ini)
+    nodes = fine.meta["nodes"]
+    ni, nj, nk = fine.blocks[0].dims
+    prim = fine.blocks[0].prim[0].reshape((7, ni + 4, nj + 4, nk + 4), order="F")[:, 2:ni + 2, 2:nj + 2, 2:nk + 2]
+    formats.write_tecplot(os.path.join(d, "INPUT", "part-ic.tec"), ["x", "y", "z"] + [f"v{m}" for m in range(7)],
+                          [formats.Zone("B1", nodes.shape[1:], nodes, prim, {})])
+    case = fine
+    for m in range(1, levels + 1):
+        name = "part-bc.txt" if m == 1 else f"part-bc{m}.txt"
+        formats.write_bc(os.path.join(d, "INPUT", name), case.reci, case.recd, case.donor, case.weight)
+        case = cases.coarsen(case)
+    with open(os.path.join(d, "INPUT", "part-properties.dat"), "w") as fh:
+        fh.write('TITLE = "Mass Thermodynamic Properties"\nVARIABLES = "Temperature", "Cp", "Density", "Enthalpy"\n')
+        fh.write('ZONE T="A"\nI=3, F=POINT\n')
+        for T in (1.0, 2.0, 3.0):
+            fh.write(f"{T} 900.0 1000.0 {900.0 * T}\n")
+
+
+def test_fortran_host_runs_the_levels(tmp_path, oracle):
+    """[ICE-Multigrid] levels = 2 through the Fortran host over the oracle's C-ABI:
+    the same field as the Python sequence (restrict, 6 coarse steps, prolong, 4 fine)"""
+    import os
+
+    from ice_amd import formats
+    from test_fortran_host import ABI, run_driver
+    fine = cases.case_box(8, IG, True)
+    d = str(tmp_path / "mg")
+    write_box_inputs(d, fine, 2, [4, 6])
+    p = run_driver(d, {"LD_LIBRARY_PATH": ABI, "OMP_NUM_THREADS": "2"})
+    assert p.returncode == 0, p.stdout[-800:] + p.stderr[-800:]
+    _, zones = formats.read_tecplot(os.path.join(d, "OUTPUT", "part-field.tec"))
+    rf, rc = cases.Runner(oracle, fine, "orc_"), cases.Runner(oracle, cases.coarsen(fine), "orc_")
+    rc.f("restrict")(rf.ctx, rc.ctx)
+    rc.step(6)
+    rc.f("prolong")(rc.ctx, rf.ctx)
+    rf.step(4)
+    want = rf.state(1, 1)
+    got = zones[0].cells.reshape(want.shape)
+    assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-30)) <= 1e-12
