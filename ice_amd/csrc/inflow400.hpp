@@ -74,7 +74,7 @@ ICE_DEV double i400_measure(const FamArgs& a, int closure, const DRec& r) {
 struct I400Chunk {
   int m, start, end, pad;
 };
-constexpr int kI400Chunk = 2048;
+constexpr int kI400Chunk = 256;
 
 // one workgroup of 256 per chunk: its records' counts and maxima reduced in the
 // workgroup, then four atomics into the boundary's accumulators
