@@ -65,15 +65,22 @@ DESCR = {
 }
 
 
-def cpu_baseline(case, sample_steps: int):
-    """The oracle (OpenMP C++ restatement) on `sample_steps` steps of the same case."""
+def cpu_baseline(case, max_steps: int, budget_s: float = 12.0):
+    """The oracle (OpenMP C++ restatement) on the same case, one step at a time,
+    until `max_steps` steps or `budget_s` seconds of CPU work: (Mcell-stage/s,
+    seconds, steps)."""
     lib = native.bind(C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so")), "orc_")
     r = cases.Runner(lib, case, "orc_")
     t0 = time.perf_counter()
-    r.step(sample_steps)
+    n = 0
+    while n < max_steps:
+        r.step(1)
+        n += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
     dt = time.perf_counter() - t0
     r.close()
-    return case.cell_stages(sample_steps) / dt / 1e6, dt
+    return case.cell_stages(n) / dt / 1e6, dt, n
 
 
 def traffic_of(workload: str, mode: str, cells_per_launch: float):
@@ -120,7 +127,8 @@ def main():
                     help="keep the default stage tile (otherwise the library's autotuner picks it, untimed)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB three times on the device (the PMC traffic calibration kernel) and exit")
-    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=50,
+                    help="at most this many steps of the CPU baseline sample, cut at about 12 s (0: skip)")
     ap.add_argument("--kernel-steps", type=int, default=2, help="steps of the per-kernel event pass")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -194,9 +202,9 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
         os.environ["OMP_NUM_THREADS"] = str(threads)
         sample = WORKLOADS[args.workload](native.MODE_PLAIN)
-        v, secs = cpu_baseline(sample, args.cpu_steps)
+        v, secs, nsteps = cpu_baseline(sample, args.cpu_steps)
         cpu = {"value": v, "unit": "Mcell-stage/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same workload from the same initial state "
+               "sample": f"{nsteps} steps of the same workload from the same initial state "
                          f"({secs:.1f} s on {threads} OpenMP threads)"}
 
     out = {
