@@ -255,8 +255,11 @@ int  ice_gpu_set_properties(ice_gpu_ctx* ctx, int p, int T_min, int n, const dou
  * domain faster than round-off, and measures it (|w| for MK, |w|/sqrt(P/rho) for IG,
  * |w|/sqrt(Pnn/rho) for AG).  A line is logged at a boundary's first such stage and
  * each time a stage's largest measure exceeds 10 times that of its last line.  The
- * device logs lines in stage order, boundaries by block then face; the host prints
- * them (the reference's text: ICE prints and flushes them at the stage). */
+ * device logs lines in the order the stages run, boundaries by block then face: the
+ * reference's order on the plain path (family by family, each family's stages in
+ * turn); the batched path runs the families of one closure group stage by stage, so
+ * their lines interleave by stage.  The host prints them (the reference's text: ICE
+ * prints and flushes them at the stage). */
 typedef struct {
   int32_t iter;             /* iter_general of the stage                                 */
   int32_t block, face, family;
