@@ -130,6 +130,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=50,
                     help="at most this many steps of the CPU baseline sample, cut at about 12 s (0: skip)")
     ap.add_argument("--kernel-steps", type=int, default=2, help="steps of the per-kernel event pass")
+    ap.add_argument("--compare-steps", type=int, default=10,
+                    help="plain mode: steps of a batched run of the same workload, reported beside (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -196,6 +198,25 @@ def main():
     cells_per_launch = case.cell_stages(args.kernel_steps) / max(launches.value, 1)
     bytes_per_launch = cells_per_launch * stage_kernel_bytes(case, args.mode == "batched")
     achieved = bytes_per_launch / avg_kernel_s / 1e9 if avg_kernel_s > 0 else None
+    run.close()
+
+    # the other path of the metric's comparison on the same workload, after the timed
+    # region: the batched path beside a plain run (a short run of its own)
+    other = None
+    if args.mode == "plain" and args.compare_steps > 0:
+        bcase = WORKLOADS[args.workload](native.MODE_BATCHED)
+        brun = cases.Runner(gpu, bcase, "ice_gpu_")
+        if not args.no_autotune:
+            best = np.zeros(4, np.int32)
+            if gpu.ice_gpu_autotune(brun.ctx, 2, best, np.zeros(16)) != 0:
+                raise RuntimeError(brun.error())
+        brun.step(2)
+        t1 = time.perf_counter()
+        brun.step(args.compare_steps)
+        bwall = time.perf_counter() - t1
+        brun.close()
+        other = {"mode": "batched", "steps": args.compare_steps,
+                 "value": bcase.cell_stages(args.compare_steps) / bwall / 1e6}
 
     cpu = None
     if args.cpu_steps > 0:
@@ -222,7 +243,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": DESCR[args.workload], "mode": args.mode, "cells": case.cells,
                    "families": case.cfg.ngroups, "rk_stages": case.cfg.nrk,
-                   "event_ms_per_step": step_event_ms / args.steps, "iter": st.iter, "autotune": tuned},
+                   "event_ms_per_step": step_event_ms / args.steps, "iter": st.iter, "autotune": tuned,
+                   "compared": other},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                      "kernel": "k_stage_tiled / k_stage_persist (fused interior flux + source + residual + RK update)",
