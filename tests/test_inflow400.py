@@ -196,3 +196,43 @@ def test_gpu_fortran_host_prints_the_report(tmp_path, mode):
     p = run_driver(d, {}, timeout=300)
     assert p.returncode == 0, p.stdout[-800:] + p.stderr[-800:]
     one_boundary(*driver_report(p.stdout), "|w|", U, 40)
+
+
+def two_family_duct(mode=0):
+    """The duct with an MK and an IG family side by side (one table for both)"""
+    mk, ig = cases.case_duct400(MK, mode=mode), cases.case_duct400(IG, mode=mode)
+    mk.cfg.ngroups = 2
+    tb = cases.TableBuilder(2)
+    codes = {1: 400, 2: 400, 3: 400, 4: 400, 5: 0, 6: 0}
+
+    def normal_dir(ri, fijk):
+        ri[:, cases.R_NORMAL_DIR] = 1
+
+    tb.add_block(1, mk.blocks[0].dims, lambda f, i, j, k: np.vectorize(codes.get)(f), normal_dir)
+    reci, recd, donor, weight = tb.arrays()
+    blk = cases.Block(mk.blocks[0].dims, mk.blocks[0].geo, [mk.blocks[0].prim[0], ig.blocks[0].prim[0]])
+    return cases.Case("duct400x2", mk.cfg, [mk.fams[0], ig.fams[0]], [blk], reci, recd, donor, weight, mk.steps)
+
+
+def test_oracle_two_families_name_each_family(oracle):
+    case = two_family_duct()
+    first, total = texts(case, *run(oracle, "orc_", case))
+    assert len(first) == 2 and len(total) == 2, first
+    assert first[0].startswith(" 400 boundary of block 1, face 2, family 1: particles entering at iteration 1")
+    assert first[1].startswith(" 400 boundary of block 1, face 2, family 2: particles entering at iteration 1")
+    assert "|w|: 1.0000E+01 m/s" in first[0] and "|w|/sqrt(P/rho): 2.0000E+00" in first[1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1], ids=["plain", "batched"])
+def test_gpu_two_families_equal_oracle(gpu, oracle, mode):
+    """plain: the reference's order; batched: the same lines, interleaved by stage"""
+    case = two_family_duct(mode)
+    gl, gt = run(gpu, "ice_gpu_", case)
+    ol, ot = run(oracle, "orc_", case)
+    key = lambda x: (x.family, x.iter, x.block, x.face)  # noqa: E731
+    if mode == 0:
+        same(gl, ol)
+    else:
+        same(sorted(gl, key=key), sorted(ol, key=key))
+    same(gt, ot)
