@@ -49,6 +49,7 @@ WORKLOADS = {
     "multiblock3d": lambda mode: cases.case_multiblock(mode=mode),
     "evap3d": lambda mode: cases.case_evaporation(n=256, steps=50, mode=mode),
     "combined": lambda mode: cases.case_combined(ntree=88, npairs=4, emin=24, emax=96, pair_n=24, mode=mode),
+    "combined_full": lambda mode: cases.case_combined(ntree=88, npairs=4, emin=32, emax=192, pair_n=24, mode=mode),
 }
 DESCR = {
     "jets2d": "2-D crossing jets, 1024x1024 cells, AG closure, HLLE, MUSCL van Leer + Jameson sensor, "
@@ -62,6 +63,9 @@ DESCR = {
     "combined": "combined configuration scaled to 10.2M cells: 88-block connection tree (edges log-uniform 24-96) + "
                 "4 chimera pairs, AG x 2 families, HLLE, full interphase physics with CEM/Langmuir-Knudsen evaporation "
                 "and Stefan blowing, gas as the evaporation case, steady SSP-RK3 with IRS (beta 0.5), FP64",
+    "combined_full": "combined configuration at full size, 48.6M cells in 96 blocks: 88-block connection tree (edges "
+                     "log-uniform 32-192) + 4 chimera pairs, AG x 2 families, HLLE, full interphase physics with "
+                     "CEM/Langmuir-Knudsen evaporation and Stefan blowing, steady SSP-RK3 with IRS (beta 0.5), FP64",
 }
 
 
@@ -147,7 +151,13 @@ def main():
         return
 
     mode = native.MODE_BATCHED if args.mode == "batched" else native.MODE_PLAIN
+    t_start = time.perf_counter()
+
+    def progress(what):
+        print(f"[bench] {time.perf_counter() - t_start:7.1f} s  {what}", file=sys.stderr, flush=True)
+
     case = WORKLOADS[args.workload](mode)
+    progress(f"case built: {case.cells} cells, {len(case.blocks)} blocks")
     gpu = native.load()
     surrogate = None
     if args.mode == "surrogate":
@@ -169,6 +179,7 @@ def main():
         surrogate["training"] = {"samples": int(rep[0]), "loss_initial": float(rep[1]), "loss_final": float(rep[2]),
                                  "rate_ratio_abs_err_mean": float(rep[3]), "rate_ratio_abs_err_max": float(rep[4]),
                                  "train_ms": float(rep[5])}
+    progress("context set up")
     tuned = None
     if not args.no_autotune:
         best = np.zeros(4, np.int32)
@@ -177,6 +188,7 @@ def main():
             raise RuntimeError(run.error())
         tuned = {"tile": best[:3].tolist(), "dispatch": "items" if best[3] else "queues",
                  "trial_ms_per_step": [round(float(x), 4) for x in trial_ms if np.isfinite(x) and x > 0]}
+    progress("autotuned" if tuned else "no autotune")
     if args.warmup:
         run.step(args.warmup)
     t0 = time.perf_counter()
@@ -187,6 +199,7 @@ def main():
     step_event_ms = step_ms.value
     # the dominant kernel, timed live with HIP events on the context's stream
     # (bracketing each launch; a separate pass so the brackets do not slow the timed region)
+    progress("timed region done")
     gpu.ice_gpu_set_kernel_timing(run.ctx, 1)
     run.step(args.kernel_steps)
     gpu.ice_gpu_set_kernel_timing(run.ctx, 0)
