@@ -9,9 +9,10 @@ oracle and on the device from the same inputs.
 * J, the time step bounded by the relaxation time (J-tau-limit/run.py:1-25): the
   first step is dt-max, the later ones tau_p, and the slip follows the SSP-RK3
   amplification of each step (below).
-* A, Stokes drag relaxation, C, periodic sine advection, and G, a cloud carried
-  by a frozen gas on a periodic line: the closed forms and observed orders of
-  those cases (below, each with its bars)."""
+* A, Stokes drag relaxation, C, periodic sine advection, G, a cloud carried by a
+  frozen gas on a periodic line, H, a cloud in a linear strain, and I, a cloud in
+  a 2-D vortex: the closed forms and observed orders of those cases (below, each
+  with its bars)."""
 import math
 
 import numpy as np
@@ -139,22 +140,25 @@ def sine_average(xl, xr, t):
     return RHO0 + AMP * (np.cos(K_C * (xl - SPEED * t)) - np.cos(K_C * (xr - SPEED * t))) / (K_C * (xr - xl))
 
 
-def periodic_line(nx, averages, u0, limiter=3, steps=1, dt_max=1.0e-4, cfl=0.8, gas_u=None, mode=0):
-    """A line of nx cells on [0, L_C] whose faces 1 and 2 are joined (201, periodic), the
-    others extrapolation (400) and type 0: MK particles of dp = 1e-4 m at the cell
-    averages `averages(xl, xr)` of density, moving at u0, at T_g; with `gas_u`, a
-    frozen gas (1.2 kg/m3, 300 K) at that speed drags them (Stokes)."""
+def periodic_line(nx, averages, u0, limiter=3, steps=1, dt_max=1.0e-4, cfl=0.8, gas_u=None, mode=0, periodic=True,
+                  dp=DP):
+    """A line of nx cells on [0, L_C] whose faces 1 and 2 are joined (201, periodic;
+    with periodic False, extrapolation 400), faces 3-4 extrapolation and 5-6 type 0:
+    MK particles of diameter dp at the cell averages `averages(xl, xr)` of density,
+    moving at u0 (a number or a function of the cell centre), at T_g; with `gas_u`
+    (likewise), a frozen gas (1.2 kg/m3, 300 K) at that speed drags them (Stokes)."""
     from ice_amd import geometry, native
     dims = (nx, 1, 1)
     dx = L_C / nx
     nodes = geometry.cartesian_nodes(np.linspace(0.0, L_C, nx + 1), [0.0, dx], [0.0, dx])
     geo = geometry.metrics(nodes)
-    vol = RHO_AL * 4.0 / 3.0 * math.pi * (0.5 * DP) ** 3
+    vol = RHO_AL * 4.0 / 3.0 * math.pi * (0.5 * dp) ** 3
+    at = lambda f, x: f(x) if callable(f) else f + 0.0 * x  # noqa: E731
 
     def fill(I, J, K):
         rho = averages((I - 1) * dx, I * dx)
         z = np.zeros(I.shape)
-        return np.stack([rho, z + u0, z, z, z + TG, rho / vol])
+        return np.stack([rho, at(u0, (I - 0.5) * dx), z, z, z + TG, rho / vol])
 
     prim = cases.padded_state(dims, 6, fill)
     tb = cases.TableBuilder(1)
@@ -168,12 +172,14 @@ def periodic_line(nx, averages, u0, limiter=3, steps=1, dt_max=1.0e-4, cfl=0.8, 
         ri[per, native.R_JS] = fijk[per, 2]
         ri[per, native.R_KS] = fijk[per, 3]
 
-    tb.add_block(1, dims, lambda f, i, j, k: np.where(f <= 2, 201, np.where(f <= 4, 400, 0)), connect)
+    side = 201 if periodic else 400
+    tb.add_block(1, dims, lambda f, i, j, k: np.where(f <= 2, side, np.where(f <= 4, 400, 0)),
+                 connect if periodic else None)
     reci, recd, donor, weight = tb.arrays()
     gas = None
     if gas_u is not None:
         g5 = np.zeros((5,) + dims)
-        g5[0], g5[1], g5[4] = 1.2, gas_u, TG
+        g5[0], g5[1], g5[4] = 1.2, at(gas_u, (np.arange(nx)[:, None, None] + 0.5) * dx), TG
         gas = cases.gas_arrays(g5, 287.05, 1.4, KG, 1.8e-5)
     cfg = cases.make_config(ngroups=1, nrk=2, time_accurate=True, ndir=1, limiter=limiter, coupled=gas is not None,
                             drag_select=2, heat_select=1, cfl=cfl, dt_max=dt_max, mode=mode)
@@ -355,3 +361,207 @@ def test_cloud_translation_oracle(oracle):
 @pytest.mark.parametrize("mode", [0, 1])
 def test_cloud_translation_gpu(gpu, mode):
     check_cloud(gpu, "ice_gpu_", mode)
+
+
+# H, a cloud in a linearly straining frozen gas u_g = U0 + a x (H-linear-strain/run.py:
+# 1-36, 86-176), seeded at the local gas velocity with n proportional to rho_p: with
+# xi = x + U0/a every particle is displaced by the same factor C(t), C = A e^{l+ t} +
+# B e^{l- t}, l± = (-1 ± sqrt(1 + 4 a tau)) / (2 tau), A + B = 1, A l+ + B l- = a, so
+# rho_p(x, t) = rho_p0(xi / C - U0/a) / C and u_p = (C'/C) xi.  To t = 0.1 s at dt =
+# 2e-5 s, extrapolation ends: the centroid within 0.05 dx, du_p/dxi within 2e-4 of
+# C'/C, the cloud's mass within 2e-3 of the exact stretch, the profile converging at
+# order >= 1.5 (nx = 100, 200, 400); then, at nx = 200, the slip coefficient a - C'/C
+# over a^2 tau (the u_g - tau Du_g/Dt asymptote) rising monotonically to 1 as St =
+# a tau falls (0.2 ... 0.025), 1 - ratio within [1.3, 2.2] St, and within 5e-2 of 1
+# at St = 0.025.
+U0_H, A_H, X0_H, SIG_H = 2.0, 6.0, 0.20, 0.035
+T_END_H, DT_H = 0.10, 2.0e-5
+
+
+def strain_profile(x):
+    return RHO_G0 * (np.exp(-(x - X0_H) ** 2 / (2.0 * SIG_H ** 2)) + FLOOR)
+
+
+def stretch(tau, t):
+    root = math.sqrt(1.0 + 4.0 * A_H * tau)
+    lp, lm = (-1.0 + root) / (2.0 * tau), (-1.0 - root) / (2.0 * tau)
+    a = (A_H - lm) / (lp - lm)
+    b = (lp - A_H) / (lp - lm)
+    return a * math.exp(lp * t) + b * math.exp(lm * t), a * lp * math.exp(lp * t) + b * lm * math.exp(lm * t)
+
+
+def strain_run(lib, prefix, nx, dp, mode=0):
+    steps = int(math.ceil(T_END_H / DT_H - 1e-9))
+    dx = L_C / nx
+    gas = lambda x: U0_H + A_H * x  # noqa: E731
+    case = periodic_line(nx, lambda xl, xr: strain_profile(0.5 * (xl + xr)), gas, 3, steps, DT_H, 0.4, gas, mode,
+                         periodic=False, dp=dp)
+    r = cases.Runner(lib, case, prefix)
+    st = r.step(steps)
+    q = r.state(1, 1)
+    rho, up = q[0][:, 0, 0], q[1][:, 0, 0]
+    xc = (np.arange(nx) + 0.5) * dx
+    xi = xc + U0_H / A_H
+    slope = (rho * up * xi).sum() / (rho * xi * xi).sum()
+    return st.time, rho, slope, xc, dx
+
+
+def check_strain(lib, prefix, mode=0):
+    tau = RHO_AL * DP ** 2 / (18.0 * MU_A)
+    errs = []
+    for nx in (100, 200, 400):
+        t, rho, slope, xc, dx = strain_run(lib, prefix, nx, DP, mode)
+        C, Cd = stretch(tau, t)
+        h = dx / 32
+        pts = (xc - 0.5 * dx)[:, None] + (np.arange(32)[None, :] + 0.5) * h
+        ref = (strain_profile((pts + U0_H / A_H) / C - U0_H / A_H) / C).mean(axis=1)
+        errs.append(np.abs(rho - ref).sum() / ref.sum())
+        xbar = (rho * xc).sum() / rho.sum()
+        assert abs(xbar - (C * (X0_H + U0_H / A_H) - U0_H / A_H)) <= 0.05 * dx, nx
+        assert abs(slope / (Cd / C) - 1.0) <= 2e-4, (nx, slope, Cd / C)
+        assert abs(rho.sum() / ref.sum() - 1.0) <= 2e-3, nx
+    errs = np.array(errs)
+    assert (np.log(errs[:-1] / errs[1:]) / np.log(2.0)).min() >= 1.5, errs
+    stokes = (0.20, 0.10, 0.05, 0.025)
+    ratios = []
+    for St in stokes:
+        tau_s = St / A_H
+        dp_s = math.sqrt(18.0 * MU_A * tau_s / RHO_AL)
+        t, rho, slope, xc, dx = strain_run(lib, prefix, 200, dp_s, mode)
+        C, Cd = stretch(tau_s, t)
+        assert abs(slope / (Cd / C) - 1.0) <= 5e-4, (St, slope, Cd / C)
+        ratios.append((A_H - slope) / (A_H ** 2 * tau_s))
+    assert all(a < b for a, b in zip(ratios[:-1], ratios[1:])), ratios
+    assert all(1.3 * s <= 1.0 - r <= 2.2 * s for r, s in zip(ratios, stokes)), ratios
+    assert abs(ratios[-1] - 1.0) <= 0.05, ratios
+
+
+def test_linear_strain_oracle(oracle):
+    check_strain(oracle, "orc_")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_linear_strain_gpu(gpu, mode):
+    check_strain(gpu, "ice_gpu_", mode)
+
+
+# I, a cloud in a frozen gas in solid-body rotation u_g = -Omega y, v_g = Omega x
+# (I-vortex-cloud/run.py:1-35, 97-200), seeded at the local gas velocity: in z = x + i y
+# every particle is displaced by the same complex factor C(t), C = A e^{l+ t} + B e^{l- t},
+# l± = (-1 ± sqrt(1 + 4 i Omega tau)) / (2 tau), A + B = 1, A l+ + B l- = i Omega, so
+# rho_p(z, t) = rho_p0(z / C) / |C|^2 and u_p + i v_p = (C'/C) z.  Over a quarter turn on
+# [-2, 2]^2 (extrapolation faces, dt-max 1e-3 s): at St = 0.1 the profile and the
+# centroid converge at order >= 1.5 (n = 48, 96, 192), the centroid within 0.1 h; over
+# St = 0.01 ... 10 at n = 96, |centroid| / x0 within 5e-3 of |C|, the lag within 5e-3 rad
+# of Omega t - arg C, the complex velocity slope within 1e-2 of C'/C, L1 <= 0.15, the
+# mass within 1e-3, the lag and the ejection growing with St, a tracer at St = 0.01 and
+# a lag >= 0.4 rad at St = 10.
+OMEGA, HALF_I, X0_I, SIG_I, FLOOR_I = 1.0, 2.0, 0.6, 0.12, 1.0e-8
+QUARTER = 0.5 * math.pi / OMEGA
+
+
+def vortex_stretch(tau, t):
+    import cmath
+    root = cmath.sqrt(1.0 + 4.0j * OMEGA * tau)
+    lp, lm = (-1.0 + root) / (2.0 * tau), (-1.0 - root) / (2.0 * tau)
+    a = (1.0j * OMEGA - lm) / (lp - lm)
+    b = (lp - 1.0j * OMEGA) / (lp - lm)
+    return a * cmath.exp(lp * t) + b * cmath.exp(lm * t), a * lp * cmath.exp(lp * t) + b * lm * cmath.exp(lm * t)
+
+
+def vortex_profile(x, y):
+    return RHO_G0 * (np.exp(-((x - X0_I) ** 2 + y ** 2) / (2.0 * SIG_I ** 2)) + FLOOR_I)
+
+
+def vortex_case(n, tau, mode=0):
+    """the square of n x n cells, the cloud and the gas at the cell centres"""
+    from ice_amd import geometry
+    h = 2.0 * HALF_I / n
+    x = np.linspace(-HALF_I, HALF_I, n + 1)
+    nodes = geometry.cartesian_nodes(x, x, [0.0, h])
+    geo = geometry.metrics(nodes)
+    dims = (n, n, 1)
+    dp = math.sqrt(18.0 * MU_A * tau / RHO_AL)
+    vol = RHO_AL * 4.0 / 3.0 * math.pi * (0.5 * dp) ** 3
+
+    def fill(I, J, K):
+        xc, yc = -HALF_I + (I - 0.5) * h, -HALF_I + (J - 0.5) * h
+        rho = vortex_profile(xc, yc)
+        z = np.zeros(I.shape)
+        return np.stack([rho, -OMEGA * yc, OMEGA * xc, z, z + TG, rho / vol])
+
+    prim = cases.padded_state(dims, 6, fill)
+    xc = -HALF_I + (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    g5 = np.zeros((5,) + dims)
+    g5[0], g5[1], g5[2], g5[4] = 1.2, -OMEGA * Y[:, :, None], OMEGA * X[:, :, None], TG
+    gas = cases.gas_arrays(g5, 287.05, 1.4, KG, MU_A)
+    tb = cases.TableBuilder(1)
+    tb.add_block(1, dims, lambda f, i, j, k: np.where(f <= 4, 400, 0))
+    reci, recd, donor, weight = tb.arrays()
+    steps = int(math.ceil(QUARTER / 1.0e-3 - 1e-9))
+    cfg = cases.make_config(ngroups=1, nrk=2, time_accurate=True, ndir=2, limiter=3, coupled=True, drag_select=2,
+                            heat_select=1, cfl=0.4, dt_max=1.0e-3, mode=mode)
+    fam = cases.make_family(MK, rho_mat=RHO_AL, cs=CS)
+    case = cases.Case("vortex", cfg, [fam], [cases.Block(dims, geo, [prim], gas)], reci, recd, donor, weight, steps)
+    return case, X + 1j * Y, h
+
+
+def vortex_run(lib, prefix, n, tau, mode=0):
+    """to a quarter turn (the last step passes it, as the reference's runs end at or
+    after t_end); the cloud's mass, centroid, complex velocity slope and L1 error"""
+    case, Z, h = vortex_case(n, tau, mode)
+    r = cases.Runner(lib, case, prefix)
+    t = r.step(case.steps).time
+    assert QUARTER <= t < QUARTER + 1.0e-3, t  # dt = dt-max on every step
+    q = r.state(1, 1)
+    rho, w = q[0][:, :, 0], q[1][:, :, 0] + 1j * q[2][:, :, 0]
+    C, Cd = vortex_stretch(tau, t)
+    mass = rho.sum()
+    centre = (rho * Z).sum() / mass
+    slope = (rho * w * np.conj(Z)).sum() / (rho * np.abs(Z) ** 2).sum()
+    z0 = Z / C
+    ref = vortex_profile(z0.real, z0.imag) / abs(C) ** 2
+    return t, C, Cd, mass, centre, slope, np.abs(rho - ref).sum() / ref.sum(), ref.sum(), h
+
+
+def check_vortex(lib, prefix, grids=(48, 96, 192), n_sweep=96, mode=0):
+    import cmath
+    errs, drifts = [], []
+    for n in grids:
+        t, C, Cd, mass, centre, slope, err, mref, h = vortex_run(lib, prefix, n, 0.1 / OMEGA, mode)
+        errs.append(err)
+        drifts.append(abs(centre - C * X0_I))
+        assert drifts[-1] <= 0.1 * h, (n, drifts[-1] / h)
+    order = lambda e: np.log(np.array(e[:-1]) / np.array(e[1:])) / np.log(2.0)  # noqa: E731
+    assert order(errs).min() >= 1.5, errs
+    assert order(drifts).min() >= 1.5, drifts
+    if n_sweep is None:
+        return
+    lags, radii = [], []
+    for St in (0.01, 0.1, 1.0, 10.0):
+        t, C, Cd, mass, centre, slope, err, mref, h = vortex_run(lib, prefix, n_sweep, St / OMEGA, mode)
+        scale, lag = abs(centre) / X0_I, OMEGA * t - cmath.phase(centre)
+        assert abs(scale / abs(C) - 1.0) <= 5e-3, (St, scale, abs(C))
+        assert abs(lag - (OMEGA * t - cmath.phase(C))) <= 5e-3, (St, lag)
+        assert abs(slope / (Cd / C) - 1.0) <= 1e-2, (St, slope, Cd / C)
+        assert err <= 0.15, (St, err)
+        assert abs(mass / mref - 1.0) <= 1e-3, (St, mass / mref)
+        lags.append(lag)
+        radii.append(scale)
+    assert all(a < b for a, b in zip(lags[:-1], lags[1:])), lags
+    assert all(a < b for a, b in zip(radii[:-1], radii[1:])), radii
+    assert lags[0] <= 5e-3 and abs(radii[0] - 1.0) <= 0.02
+    assert lags[-1] >= 0.4
+
+
+def test_vortex_cloud_oracle(oracle):
+    """the refinement leg on the two coarser grids (the full case takes the oracle minutes)"""
+    check_vortex(oracle, "orc_", grids=(48, 96), n_sweep=None)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_vortex_cloud_gpu(gpu, mode):
+    check_vortex(gpu, "ice_gpu_", mode=mode)
