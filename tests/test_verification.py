@@ -565,3 +565,70 @@ def test_vortex_cloud_oracle(oracle):
 @pytest.mark.parametrize("mode", [0, 1])
 def test_vortex_cloud_gpu(gpu, mode):
     check_vortex(gpu, "ice_gpu_", mode=mode)
+
+
+# L1, each family on its own material (L-materials/run.py:8-17, 68-72, 156-194): MK
+# families with the same (rho_p, n) at rest and 300 K in a gas at rest at 400 K, Stokes
+# heat (Nu = 2), no drag.  Each family recovers Rp from its own material's density and
+# relaxes with tau = rho_p cp / (4 pi k Rp n); materials A (cp 1000, rho 2000) and B
+# (cp 2000, rho 1000): tau_B / tau_A = 2 (1/2)^(1/3).  To t = tau_A at dt = tau_min / 1000,
+# every family within 1e-6 of the gap (one heating as the other material misses by 0.165).
+MAT_A, MAT_B = (1000.0, 2000.0), (2000.0, 1000.0)   # (cp, rho)
+
+
+def materials_case(mats, mode=0):
+    from ice_amd import geometry
+    nf = len(mats)
+    nx = 8
+    dims = (nx, 1, 1)
+    nodes = geometry.cartesian_nodes(np.linspace(0.0, 1.0, nx + 1), [0.0, 1.0 / nx], [0.0, 1.0 / nx])
+    geo = geometry.metrics(nodes)
+    rho_p = ALPHA * 2000.0
+    n = rho_p / (2000.0 * 4.0 / 3.0 * math.pi * (0.5 * DP) ** 3)
+    taus = []
+    for cp, rho_m in mats:
+        Rp = (0.75 * rho_p / (n * math.pi * rho_m)) ** (1.0 / 3.0)
+        taus.append(rho_p * cp / (4.0 * math.pi * KG * Rp * n))
+    dt = min(taus) / 1000.0
+    steps = int(math.ceil(taus[0] / dt - 1e-9))
+
+    def fill(I, J, K):
+        z = np.zeros(I.shape)
+        return np.stack([z + rho_p, z, z, z, z + TG, z + n])
+
+    prims = [cases.padded_state(dims, 6, fill) for _ in range(nf)]
+    g5 = np.zeros((5,) + dims)
+    g5[0], g5[4] = 1.2, 400.0
+    gas = cases.gas_arrays(g5, 287.05, 1.4, KG, MU_A)
+    tb = cases.TableBuilder(nf)
+    tb.add_block(1, dims, lambda f, i, j, k: np.where(f <= 4, 400, 0))
+    reci, recd, donor, weight = tb.arrays()
+    cfg = cases.make_config(ngroups=nf, nrk=2, time_accurate=True, ndir=1, limiter=3, coupled=True, drag_select=13,
+                            heat_select=1, cfl=0.8, dt_max=dt, mode=mode)
+    fams = [cases.make_family(MK, rho_mat=rho_m, cs=cp) for cp, rho_m in mats]
+    case = cases.Case("materials", cfg, fams, [cases.Block(dims, geo, prims, gas)], reci, recd, donor, weight, steps)
+    return case, taus
+
+
+def check_materials(lib, prefix, mats, mode=0):
+    case, taus = materials_case(mats, mode)
+    r = cases.Runner(lib, case, prefix)
+    t = r.step(case.steps).time
+    for p in range(len(mats)):
+        T = r.state(1, p + 1)[4]
+        ref = 400.0 + (TG - 400.0) * math.exp(-t / taus[p])
+        assert np.max(np.abs(T - ref)) <= 1e-6 * (400.0 - TG), (p, float(T[0, 0, 0]), ref)
+    other = 400.0 + (TG - 400.0) * math.exp(-t / taus[-1])
+    assert abs(r.state(1, 1)[4][0, 0, 0] - other) >= 0.1 * (400.0 - TG) or mats[0] == mats[-1]
+
+
+@pytest.mark.parametrize("mats", [(MAT_A, MAT_B), (MAT_A, MAT_A, MAT_B)], ids=["A1-B1", "A2-B1"])
+def test_materials_oracle(oracle, mats):
+    check_materials(oracle, "orc_", mats)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mats", [(MAT_A, MAT_B), (MAT_A, MAT_A, MAT_B)], ids=["A1-B1", "A2-B1"])
+def test_materials_gpu(gpu, mats, mode):
+    check_materials(gpu, "ice_gpu_", mats, mode)
