@@ -76,6 +76,8 @@ struct ice_gpu_ctx {
   std::vector<ice_family_t> fams;
   std::vector<BlkDev> blk;
   BlkDev* d_blk = nullptr;
+  int* d_irs_wg = nullptr;   // first workgroup of each block in the all-blocks IRS launch, then the total
+  int irs_nwg = 0;
   long long N = 0;
   double *vol = nullptr, *dl = nullptr, *M = nullptr, *fa = nullptr;
   double *gas = nullptr, *gR = nullptr, *gG = nullptr, *gK = nullptr, *gMu = nullptr, *yinf = nullptr;
@@ -509,6 +511,7 @@ void dispatch_stage(ice_gpu_ctx* c, FamArgs& a) {
   L.grp = F.grp; L.ngrp = F.ngrp; L.own = F.own; L.nown = F.nown; L.nbcell = F.nbcell;
   L.riemann = c->fams[a.fam].riemann; L.sd = c->cfg.shock_detector; L.coupled = c->cfg.coupled;
   L.irs = c->cfg.irs; L.limiter = c->cfg.limiter; L.ndir = c->cfg.ndir;
+  L.irs_wg = c->d_irs_wg; L.irs_nwg = c->irs_nwg;
   L.timing = c->kernel_timing; L.kev = &c->kev;
   int cl = c->fams[a.fam].closure;
   if (cl == ICE_MK) launch_stage_mk(L, a);
@@ -616,6 +619,13 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
   const double nan = std::numeric_limits<double>::quiet_NaN();
   if (dalloc(c, &c->d_blk, nb)) return 1;
   HIPCHK(hipMemcpy(c->d_blk, c->blk.data(), nb * sizeof(BlkDev), hipMemcpyHostToDevice));
+  {
+    std::vector<int> wg(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) wg[b + 1] = wg[b] + (int)grid_cells(c->blk[b].cells);
+    c->irs_nwg = wg[nb];
+    if (dalloc(c, &c->d_irs_wg, nb + 1)) return 1;
+    HIPCHK(hipMemcpy(c->d_irs_wg, wg.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+  }
   if (dalloc(c, &c->vol, N) || dalloc(c, &c->dl, 3 * N) || dalloc(c, &c->M, 9 * N) || dalloc(c, &c->fa, 12 * N))
     return 1;
   if (fill(c, c->vol, N, nan) || fill(c, c->dl, 3 * N, nan) || fill(c, c->M, 9 * N, nan) || fill(c, c->fa, 12 * N, nan))
@@ -1360,7 +1370,7 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
   auto F = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  F(c->d_blk); F(c->vol); F(c->dl); F(c->M); F(c->fa); F(c->gas); F(c->gR); F(c->gG); F(c->gK); F(c->gMu);
+  F(c->d_blk); F(c->d_irs_wg); F(c->vol); F(c->dl); F(c->M); F(c->fa); F(c->gas); F(c->gR); F(c->gG); F(c->gK); F(c->gMu);
   F(c->yinf); F(c->d_rec); F(c->d_donor); F(c->d_weight); F(c->d_st);
   for (auto& f : c->fam) {
     for (int m = 0; m < 3; ++m) F(f.buf[m]);
@@ -1535,6 +1545,8 @@ BatchLaunch group_launch(ice_gpu_ctx* c, const BatchGroup& g, const std::vector<
   B.nitems = g.nitems;
   B.irs = c->cfg.irs;
   B.ndir = c->cfg.ndir;
+  B.irs_wg = c->d_irs_wg;
+  B.irs_nwg = c->irs_nwg;
   B.nb = (int)c->blk.size();
   B.blk = c->blk.data();
   B.riemann = g.riemann;

@@ -20,11 +20,29 @@ struct StageLaunch {
   int nown;
   int nbcell;
   int riemann, sd, coupled, irs, limiter, ndir;
+  const int* irs_wg;        // first workgroup of each block in the all-blocks IRS launch
+  int irs_nwg;              // workgroups of that launch
   int timing;                        // bracket each stage launch with events
   std::vector<hipEvent_t>* kev;
 };
 
 inline unsigned grid_cells(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+// Implicit residual smoothing of a family over every block: one launch per direction,
+// both Jacobi sweeps fused, from R into T, then T into R, and so on.  Returns the
+// buffer that holds the smoothed residual.
+template <int NC>
+const double* launch_irs(const FamArgs& a, int ndir, const int* wg, int nwg, int nb, hipStream_t stream) {
+  const double* src = a.R;
+  double* dst = a.T;
+  for (int d = 0; d < ndir; ++d) {
+    hipLaunchKernelGGL(k_irs<NC>, dim3((unsigned)nwg), dim3(kThreads), 0, stream, a, d, src, dst, wg, nb);
+    double* t = const_cast<double*>(src);
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
 
 template <int M, int S, bool SD, bool CP>
 void launch_tiled(const StageLaunch& L, FamArgs& a, int b) {
@@ -85,12 +103,9 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
     }
   }
   if (L.irs) {
-    for (int d = 0; d < L.ndir; ++d)
-      for (int sw = 1; sw <= 2; ++sw)
-        for (int b = 0; b < nb; ++b)
-          hipLaunchKernelGGL(k_irs<NC>, dim3(grid_cells(L.blk[b].cells)), dim3(kThreads), 0, L.stream, a, b, d, sw);
+    const double* res = launch_irs<NC>(a, L.ndir, L.irs_wg, L.irs_nwg, nb, L.stream);
     for (int b = 0; b < nb; ++b)
-      hipLaunchKernelGGL(k_update<M>, dim3(grid_cells(L.blk[b].cells)), dim3(kThreads), 0, L.stream, a, b);
+      hipLaunchKernelGGL(k_update<M>, dim3(grid_cells(L.blk[b].cells)), dim3(kThreads), 0, L.stream, a, b, res);
   }
 }
 
@@ -121,7 +136,9 @@ struct BatchLaunch {
   int grid;                    // persistent workgroups
   int dispatch;                // 0: persistent workgroups over the queues, 1: one workgroup per item
   int nitems;
-  int irs, ndir, nb;           // implicit residual smoothing: the stage writes R, then per block sweeps
+  int irs, ndir, nb;           // implicit residual smoothing: the stage writes R, then the sweeps
+  const int* irs_wg;           // first workgroup of each block in the all-blocks IRS launch
+  int irs_nwg;
   const BlkDev* blk;           // host copies of the blocks
   int riemann, sd, coupled, limiter;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // recorded around the persistent stage kernel when set
@@ -170,12 +187,9 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
     constexpr int NC = Closure<M>::NC;
     for (int f = 0; f < B.nfam; ++f) {
       FamArgs a = B.h_fams[B.fam_ids[f]];
-      for (int d = 0; d < B.ndir; ++d)
-        for (int sw = 1; sw <= 2; ++sw)
-          for (int b = 0; b < B.nb; ++b)
-            hipLaunchKernelGGL(k_irs<NC>, dim3(grid_cells(B.blk[b].cells)), blk, 0, B.stream, a, b, d, sw);
+      const double* res = launch_irs<NC>(a, B.ndir, B.irs_wg, B.irs_nwg, B.nb, B.stream);
       for (int b = 0; b < B.nb; ++b)
-        hipLaunchKernelGGL(k_update<M>, dim3(grid_cells(B.blk[b].cells)), blk, 0, B.stream, a, b);
+        hipLaunchKernelGGL(k_update<M>, dim3(grid_cells(B.blk[b].cells)), blk, 0, B.stream, a, b, res);
     }
     return;
   }

@@ -619,13 +619,26 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage(FamArg
 }
 
 // ------------------------------------------------------------------ IRS
-// residual_smoothing_ (Lib_IRS.f90:34-86), direction d: the first Jacobi sweep
-// into T, the second back into R; ghosts are the zero-gradient copies of the
-// residual the direction started from.
+// residual_smoothing_ (Lib_IRS.f90:34-86), direction d, over every block of the
+// family in one launch (workgroups wg[b] .. wg[b+1]-1 take block b).  The
+// reference's two Jacobi sweeps are fused: the first sweep's values at m-1 and
+// m+1 are formed in registers by the same expression the first sweep applies
+// there, so the second sweep reads exactly the doubles it would read from T.
+// Ghosts are the zero-gradient copies of the residual the direction started
+// from (src).  The result goes to dst, a buffer other than src.
 template <int NC>
-__global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int b, int d, int sweep) {
+__global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int d, const double* __restrict__ src,
+                                                  double* __restrict__ dst, const int* __restrict__ wg, int nb) {
+  int lo = 0, hi = nb - 1;
+  const int w = (int)blockIdx.x;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (wg[mid] <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  const int b = lo;
   const BlkDev B = a.blk[b];
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long t = (long long)(w - wg[b]) * blockDim.x + threadIdx.x;
   if (t >= B.cells) return;
   int i, j, k;
   cell_of(B, t, i, j, k);
@@ -633,21 +646,30 @@ __global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int b, int d, int s
   long long s = dir_stride(B, d);
   int m = d == 0 ? i : (d == 1 ? j : k);
   int n = d == 0 ? B.ni : (d == 1 ? B.nj : B.nk);
-  const double* star = sweep == 1 ? a.R : a.T;
-  double* out = sweep == 1 ? a.T : a.R;
-  double den = 1.0 + 2.0 * a.irs_beta;
+  const double beta = a.irs_beta;
+  const double den = 1.0 + 2.0 * beta;
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
-    long long o = (long long)v * a.N;
-    double r0 = a.R[o + g];
-    double lo = m > 1 ? star[o + g - s] : r0;
-    double hi = m < n ? star[o + g + s] : r0;
-    out[o + g] = (r0 + a.irs_beta * (lo + hi)) / den;
+    const double* R = src + (long long)v * a.N;
+    const double r0 = R[g];
+    double tm = r0, tp = r0;  // the first sweep's values at m-1 and m+1, or the ghost copies
+    if (m > 1) {
+      const double rm = R[g - s];
+      const double rmm = m - 1 > 1 ? R[g - 2 * s] : rm;
+      tm = (rm + beta * (rmm + r0)) / den;
+    }
+    if (m < n) {
+      const double rp = R[g + s];
+      const double rpp = m + 1 < n ? R[g + 2 * s] : rp;
+      tp = (rp + beta * (r0 + rpp)) / den;
+    }
+    dst[(long long)v * a.N + g] = (r0 + beta * (tm + tp)) / den;
   }
 }
 
+// the update of block b from the smoothed residual res
 template <int M>
-__global__ void __launch_bounds__(kThreads) k_update(FamArgs a, int b) {
+__global__ void __launch_bounds__(kThreads) k_update(FamArgs a, int b, const double* __restrict__ res) {
   constexpr int NC = Closure<M>::NC;
   const BlkDev& B = a.blk[b];
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -656,7 +678,7 @@ __global__ void __launch_bounds__(kThreads) k_update(FamArgs a, int b) {
   cell_of(B, t, i, j, k);
   long long g = gidx(B, i, j, k);
   double r[NC];
-  load_cell<NC>(a.R, a.N, g, r);
+  load_cell<NC>(res, a.N, g, r);
   update_cell<M>(a, B, i, j, k, g, r);
 }
 
