@@ -1,5 +1,5 @@
-"""Evaporating drops against published closed forms: parts 1 and 3 of ICE's
-verification case F (test/verification/F-evaporation/run.py:1-40, 392-477), on the
+"""Evaporating drops against published closed forms: parts 1, 3 and 4 of ICE's
+verification case F (test/verification/F-evaporation/run.py:1-40, 392-497), on the
 oracle and on the device.
 
 Water drops (d0 = 1e-4 m, 1000 kg/m3, alpha = 1e-3) at rest at 300 K in air at rest
@@ -21,6 +21,10 @@ Part 3, nothing frozen (cs = 4180): with the d2-law, Nu = 2 and the Miller-Harst
 Bellan blowing factor, b = ln(1 + B_T) identically, so the blown convective heat
 equals the latent sink term by term and the drop holds 300 K (to 1e-8 K) while its
 d^2 falls at the d2-law's K (1e-6); without the blowing factor it heats by over 1 K.
+
+Part 4, the same slopes at 380 K, above the boiling point: psat > p puts X_s on the
+cap 1 - 1e-12 (B_M about 6e11), to 1e-5 (one ulp of Y_s at the cap moves the rate
+by 2.5e-6), over the time CEM takes to lose 10 % of the mass.
 
 The TC model's slope needs its film equation solved and is left to the bit-for-bit
 tests against the reference's Lib_Evaporation."""
@@ -62,13 +66,13 @@ def diameter(rho_p):
     return 2.0 * (0.75 * rho_p / (NDENS * math.pi * RHO_L)) ** (1.0 / 3.0)
 
 
-def drop_case(evap, cs, t_end, dt, blow=0, mode=0):
+def drop_case(evap, cs, t_end, dt, blow=0, mode=0, tp0=TP0):
     """a line of 8 cells (extrapolation faces) holding the drops, RK2 at dt, Stokes drag and heat"""
     nx = 8
     dims = (nx, 1, 1)
     geo = geometry.metrics(geometry.cartesian_nodes(np.linspace(0.0, 1.0, nx + 1), [0.0, 0.125], [0.0, 0.125]))
     prim = cases.padded_state(dims, 6, lambda I, J, K: np.stack(
-        [np.full(I.shape, RHO_P0), 0 * I, 0 * I, 0 * I, np.full(I.shape, TP0), np.full(I.shape, NDENS)]))
+        [np.full(I.shape, RHO_P0), 0 * I, 0 * I, 0 * I, np.full(I.shape, tp0), np.full(I.shape, NDENS)]))
     g5 = np.zeros((5,) + dims)
     g5[0], g5[4] = RHO_G, TG
     gas = cases.gas_arrays(g5, RGAS, GAM, KG, MU)
@@ -83,8 +87,8 @@ def drop_case(evap, cs, t_end, dt, blow=0, mode=0):
     return cases.Case("drops", cfg, [fam], [cases.Block(dims, geo, [prim], gas)], reci, recd, donor, weight, steps)
 
 
-def run(lib, prefix, evap, cs, t_end, dt, blow=0, mode=0):
-    case = drop_case(evap, cs, t_end, dt, blow, mode)
+def run(lib, prefix, evap, cs, t_end, dt, blow=0, mode=0, tp0=TP0):
+    case = drop_case(evap, cs, t_end, dt, blow, mode, tp0)
     r = cases.Runner(lib, case, prefix)
     t = r.step(case.steps).time
     q = r.state(1, 1)
@@ -104,6 +108,21 @@ def frozen_slopes(lib, prefix, mode=0):
     assert not bad, bad
     t, q = run(lib, prefix, 0, 1.0e12, 0.04, 2.0e-5, mode=mode)
     assert np.max(np.abs(q[0] - RHO_P0)) <= 1e-14 * RHO_P0
+
+
+def boiling_slopes(lib, prefix, mode=0):
+    """part 4: at 380 K psat exceeds p, so X_s sits on the cap 1 - 1e-12; the run lasts
+    until CEM has lost 10 % of its mass, at 1000 steps"""
+    tb = 380.0
+    t_end = (1.0 - 0.9 ** (2.0 / 3.0)) * D0 ** 2 / slope("CEM", tb)
+    bad = []
+    for model, evap in MODELS.items():
+        t, q = run(lib, prefix, evap, 1.0e12, t_end, t_end / 1000.0, mode=mode, tp0=tb)
+        K = (D0 ** 2 - diameter(q[0][0, 0, 0]) ** 2) / t
+        err = abs(K - slope(model, tb)) / slope(model, tb)
+        if err > 1e-5:
+            bad.append((model, K, slope(model, tb), err))
+    assert not bad, bad
 
 
 def isothermal(lib, prefix, mode=0):
@@ -127,6 +146,10 @@ def test_frozen_slopes_oracle(oracle):
     frozen_slopes(oracle, "orc_")
 
 
+def test_boiling_slopes_oracle(oracle):
+    boiling_slopes(oracle, "orc_")
+
+
 def test_isothermal_blowing_oracle(oracle):
     isothermal(oracle, "orc_")
 
@@ -135,6 +158,12 @@ def test_isothermal_blowing_oracle(oracle):
 @pytest.mark.parametrize("mode", [0, 1])
 def test_frozen_slopes_gpu(gpu, mode):
     frozen_slopes(gpu, "ice_gpu_", mode)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_boiling_slopes_gpu(gpu, mode):
+    boiling_slopes(gpu, "ice_gpu_", mode)
 
 
 @pytest.mark.gpu
