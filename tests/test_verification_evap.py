@@ -1,5 +1,5 @@
-"""Evaporating drops against published closed forms: parts 1, 3 and 4 of ICE's
-verification case F (test/verification/F-evaporation/run.py:1-40, 392-497), on the
+"""Evaporating drops against published closed forms: parts 1, 3, 4 and 5 of ICE's
+verification case F (test/verification/F-evaporation/run.py:1-40, 392-530), on the
 oracle and on the device.
 
 Water drops (d0 = 1e-4 m, 1000 kg/m3, alpha = 1e-3) at rest at 300 K in air at rest
@@ -26,6 +26,10 @@ Part 4, the same slopes at 380 K, above the boiling point: psat > p puts X_s on 
 cap 1 - 1e-12 (B_M about 6e11), to 1e-5 (one ulp of Y_s at the cap moves the rate
 by 2.5e-6), over the time CEM takes to lose 10 % of the mass.
 
+Part 5, psat from a property table's column instead of the curve, the drop held on
+the 300 K row (cp = 1e16): CEM's slope with the column's value, 1.2 times the curve
+(a rate 1.207 times higher) and the curve itself (the control), to 1e-6.
+
 The TC model's slope needs its film equation solved and is left to the bit-for-bit
 tests against the reference's Lib_Evaporation."""
 import math
@@ -47,8 +51,12 @@ NDENS = RHO_P0 / (RHO_L * 4.0 / 3.0 * math.pi * (0.5 * D0) ** 3)
 MODELS = {"d2-law": 1, "CEM": 2, "CEM-B": 3, "ASM": 4}
 
 
-def slope(model, Tp=TP0):
-    psat = PATM * math.exp(-(LV * MV / RU) * (1.0 / Tp - 1.0 / TBOIL))
+def psat_cc(Tp):
+    return PATM * math.exp(-(LV * MV / RU) * (1.0 / Tp - 1.0 / TBOIL))
+
+
+def slope(model, Tp=TP0, factor=1.0):
+    psat = factor * psat_cc(Tp)
     Xs = min(psat / (RHO_G * RGAS * TG), 1.0 - 1.0e-12)
     Ys = Xs * MV / (Xs * MV + (1.0 - Xs) * MGAS)
     BM = Ys / (1.0 - Ys)
@@ -66,7 +74,7 @@ def diameter(rho_p):
     return 2.0 * (0.75 * rho_p / (NDENS * math.pi * RHO_L)) ** (1.0 / 3.0)
 
 
-def drop_case(evap, cs, t_end, dt, blow=0, mode=0, tp0=TP0):
+def drop_case(evap, cs, t_end, dt, blow=0, mode=0, tp0=TP0, table=None):
     """a line of 8 cells (extrapolation faces) holding the drops, RK2 at dt, Stokes drag and heat"""
     nx = 8
     dims = (nx, 1, 1)
@@ -84,11 +92,12 @@ def drop_case(evap, cs, t_end, dt, blow=0, mode=0, tp0=TP0):
                             heat_select=1, blow_select=blow, cfl=0.8, dt_max=dt, mode=mode)
     fam = cases.make_family(native.MK, rho_mat=RHO_L, cs=cs, lv=LV, evap=evap, intf=0,
                             ep=cases.water_ep(yinf=0.0, alphae=1.0))
-    return cases.Case("drops", cfg, [fam], [cases.Block(dims, geo, [prim], gas)], reci, recd, donor, weight, steps)
+    return cases.Case("drops", cfg, [fam], [cases.Block(dims, geo, [prim], gas)], reci, recd, donor, weight, steps,
+                      meta={"tables": {1: table}} if table is not None else {})
 
 
-def run(lib, prefix, evap, cs, t_end, dt, blow=0, mode=0, tp0=TP0):
-    case = drop_case(evap, cs, t_end, dt, blow, mode, tp0)
+def run(lib, prefix, evap, cs, t_end, dt, blow=0, mode=0, tp0=TP0, table=None):
+    case = drop_case(evap, cs, t_end, dt, blow, mode, tp0, table)
     r = cases.Runner(lib, case, prefix)
     t = r.step(case.steps).time
     q = r.state(1, 1)
@@ -125,6 +134,24 @@ def boiling_slopes(lib, prefix, mode=0):
     assert not bad, bad
 
 
+def psat_column(lib, prefix, mode=0):
+    """part 5: psat from the table's column (250..450 K) in place of the curve, the drop
+    held on the 300 K row by cp = 1e16; CEM's slope follows the column"""
+    from ice_amd import properties
+    T = np.arange(250.0, 451.0)
+    cp = np.full(T.size, 1.0e16)
+    Ks = []
+    for factor in (1.2, 1.0):
+        tab = properties.material_table(T, cp, np.full(T.size, RHO_L), cp * T,
+                                        psat=factor * np.array([psat_cc(x) for x in T]), evaporates=True)
+        t, q = run(lib, prefix, MODELS["CEM"], 1.0e16, 0.04, 2.0e-5, mode=mode, table=tab)
+        K = (D0 ** 2 - diameter(q[0][0, 0, 0]) ** 2) / t
+        assert abs(K - slope("CEM", factor=factor)) <= 1e-6 * slope("CEM", factor=factor), (factor, K)
+        Ks.append(K)
+    assert abs(Ks[0] / Ks[1] - slope("CEM", factor=1.2) / slope("CEM")) <= 1e-6
+    assert Ks[0] / Ks[1] > 1.2
+
+
 def isothermal(lib, prefix, mode=0):
     t, q = run(lib, prefix, 1, CS_L, 0.05, 5.0e-6, blow=1, mode=mode)
     assert np.max(np.abs(q[4] - TP0)) <= 1e-8, np.max(np.abs(q[4] - TP0))
@@ -150,6 +177,10 @@ def test_boiling_slopes_oracle(oracle):
     boiling_slopes(oracle, "orc_")
 
 
+def test_psat_column_oracle(oracle):
+    psat_column(oracle, "orc_")
+
+
 def test_isothermal_blowing_oracle(oracle):
     isothermal(oracle, "orc_")
 
@@ -164,6 +195,12 @@ def test_frozen_slopes_gpu(gpu, mode):
 @pytest.mark.parametrize("mode", [0, 1])
 def test_boiling_slopes_gpu(gpu, mode):
     boiling_slopes(gpu, "ice_gpu_", mode)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_psat_column_gpu(gpu, mode):
+    psat_column(gpu, "ice_gpu_", mode)
 
 
 @pytest.mark.gpu
