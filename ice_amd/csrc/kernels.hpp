@@ -66,7 +66,7 @@ struct FamArgs {
   uint8_t* inj;         // [N]
   double* src;          // [nc][N] kept source (optional)
   double* R;            // [nc][N] materialised residual (IRS)
-  double* T;            // [nc][N] IRS sweep buffer
+  double* T;            // [nc][N] IRS buffer: the directions ping-pong between R and T
   double* bsum;         // [nbcell][nc] boundary-flux sums
   const int* bidx;      // [N] boundary cell of a surface cell, -1 none
   double* connF;        // [nconn][nc]
