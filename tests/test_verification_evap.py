@@ -30,6 +30,9 @@ Part 5, psat from a property table's column instead of the curve, the drop held 
 the 300 K row (cp = 1e16): CEM's slope with the column's value, 1.2 times the curve
 (a rate 1.207 times higher) and the curve itself (the control), to 1e-6.
 
+And ICE's case L2: two families side by side, one evaporating (CEM) and one not, each
+on its own model: the first follows CEM's slope, the second keeps its state bit for bit.
+
 The TC model's slope needs its film equation solved and is left to the bit-for-bit
 tests against the reference's Lib_Evaporation."""
 import math
@@ -152,6 +155,27 @@ def psat_column(lib, prefix, mode=0):
     assert Ks[0] / Ks[1] > 1.2
 
 
+def per_family_models(lib, prefix, mode=0):
+    """ICE's case L2 (L-materials/run.py:18-24): two families side by side, A with CEM and B
+    with no evaporation: A follows CEM's slope, B keeps rho_p and n to the last bit"""
+    case = drop_case(MODELS["CEM"], 1.0e12, 0.04, 2.0e-5, mode=mode)
+    fam_b = cases.make_family(native.MK, rho_mat=RHO_L, cs=1.0e12, lv=LV, evap=0, intf=0,
+                              ep=cases.water_ep(yinf=0.0, alphae=1.0))
+    case.fams.append(fam_b)
+    case.cfg.ngroups = 2
+    B = case.blocks[0]
+    B.prim = [B.prim[0], B.prim[0].copy()]
+    tb = cases.TableBuilder(2)
+    tb.add_block(1, B.dims, lambda f, i, j, k: np.where(f <= 4, 400, 0))
+    case.reci, case.recd, case.donor, case.weight = tb.arrays()
+    r = cases.Runner(lib, case, prefix)
+    t = r.step(case.steps).time
+    qa, qb = r.state(1, 1), r.state(1, 2)
+    K = (D0 ** 2 - diameter(qa[0][0, 0, 0]) ** 2) / t
+    assert abs(K - slope("CEM")) <= 1e-6 * slope("CEM"), K
+    assert np.all(qb[0] == RHO_P0) and np.all(qb[5] == NDENS)
+
+
 def isothermal(lib, prefix, mode=0):
     t, q = run(lib, prefix, 1, CS_L, 0.05, 5.0e-6, blow=1, mode=mode)
     assert np.max(np.abs(q[4] - TP0)) <= 1e-8, np.max(np.abs(q[4] - TP0))
@@ -181,6 +205,10 @@ def test_psat_column_oracle(oracle):
     psat_column(oracle, "orc_")
 
 
+def test_per_family_models_oracle(oracle):
+    per_family_models(oracle, "orc_")
+
+
 def test_isothermal_blowing_oracle(oracle):
     isothermal(oracle, "orc_")
 
@@ -201,6 +229,12 @@ def test_boiling_slopes_gpu(gpu, mode):
 @pytest.mark.parametrize("mode", [0, 1])
 def test_psat_column_gpu(gpu, mode):
     psat_column(gpu, "ice_gpu_", mode)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_per_family_models_gpu(gpu, mode):
+    per_family_models(gpu, "ice_gpu_", mode)
 
 
 @pytest.mark.gpu
