@@ -583,6 +583,11 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
     *out = c;
     return 1;
   }
+  if (cfg->ndir < 1 || cfg->ndir > 3) {
+    fail(c, "[ICE::create] ndir (the mesh's resolved directions) must be 1, 2 or 3");
+    *out = c;
+    return 1;
+  }
   if (cfg->irs && cfg->time_accurate) {
     fail(c, "[ICE::Assign_Setup] implicit residual smoothing is for steady runs (time-accurate = false)");
     *out = c;

@@ -626,26 +626,21 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage(FamArg
 // there, so the second sweep reads exactly the doubles it would read from T.
 // Ghosts are the zero-gradient copies of the residual the direction started
 // from (src).  The result goes to dst, a buffer other than src.
-template <int NC>
-__global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int d, const double* __restrict__ src,
-                                                  double* __restrict__ dst, const int* __restrict__ wg, int nb) {
+// the block of workgroup w of an all-blocks launch (workgroups wg[b] .. wg[b+1]-1 take block b)
+ICE_DEV int block_of_workgroup(const int* __restrict__ wg, int nb, int w) {
   int lo = 0, hi = nb - 1;
-  const int w = (int)blockIdx.x;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
     if (wg[mid] <= w) lo = mid;
     else hi = mid - 1;
   }
-  const int b = lo;
-  const BlkDev B = a.blk[b];
-  long long t = (long long)(w - wg[b]) * blockDim.x + threadIdx.x;
-  if (t >= B.cells) return;
-  int i, j, k;
-  cell_of(B, t, i, j, k);
-  long long g = gidx(B, i, j, k);
-  long long s = dir_stride(B, d);
-  int m = d == 0 ? i : (d == 1 ? j : k);
-  int n = d == 0 ? B.ni : (d == 1 ? B.nj : B.nk);
+  return lo;
+}
+
+// the value direction d leaves at cell g (index m of n along d), for every slot
+template <int NC>
+ICE_DEV void irs_line(const FamArgs& a, const double* __restrict__ src, long long g, long long s, int m, int n,
+                      double* out) {
   const double beta = a.irs_beta;
   const double den = 1.0 + 2.0 * beta;
 #pragma unroll
@@ -663,8 +658,27 @@ __global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int d, const double
       const double rpp = m + 1 < n ? R[g + 2 * s] : rp;
       tp = (rp + beta * (r0 + rpp)) / den;
     }
-    dst[(long long)v * a.N + g] = (r0 + beta * (tm + tp)) / den;
+    out[v] = (r0 + beta * (tm + tp)) / den;
   }
+}
+
+template <int NC>
+__global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int d, const double* __restrict__ src,
+                                                  double* __restrict__ dst, const int* __restrict__ wg, int nb) {
+  const int w = (int)blockIdx.x;
+  const int b = block_of_workgroup(wg, nb, w);
+  const BlkDev B = a.blk[b];
+  long long t = (long long)(w - wg[b]) * blockDim.x + threadIdx.x;
+  if (t >= B.cells) return;
+  int i, j, k;
+  cell_of(B, t, i, j, k);
+  long long g = gidx(B, i, j, k);
+  int m = d == 0 ? i : (d == 1 ? j : k);
+  int n = d == 0 ? B.ni : (d == 1 ? B.nj : B.nk);
+  double r[NC];
+  irs_line<NC>(a, src, g, dir_stride(B, d), m, n, r);
+#pragma unroll
+  for (int v = 0; v < NC; ++v) dst[(long long)v * a.N + g] = r[v];
 }
 
 // the update of block b from the smoothed residual res
