@@ -67,7 +67,7 @@ typedef struct {
   int32_t nrk;              /* 1 euler, 2 RK2, 3 RK3                                     */
   int32_t time_accurate;    /* 1: one global dt (min over cells, families)               */
   int32_t cfl_ramp_iter;    /* cfl-rampa-iter                                            */
-  int32_t ndir;             /* resolved directions (setup_metrics)                       */
+  int32_t ndir;             /* resolved directions, 1..3 (setup_metrics)                 */
   int32_t limiter;          /* ICE_LIM_*                                                 */
   int32_t shock_detector;   /* 1: Jameson sensor weight on the slopes                    */
   int32_t irs;              /* 1: implicit residual smoothing                            */
