@@ -30,6 +30,20 @@ class Block:
     yinf: Optional[np.ndarray] = None
 
 
+class LazyBlock:
+    """A block whose arrays are made only when a Runner uploads it (make() returns
+    the Block), so that a large case never holds every block's arrays at once."""
+
+    def __init__(self, dims, make):
+        self.dims = tuple(dims)
+        self.make = make
+
+
+def block_data(B) -> Block:
+    """The arrays of a block, made now if it is a LazyBlock."""
+    return B.make() if isinstance(B, LazyBlock) else B
+
+
 @dataclasses.dataclass
 class Case:
     name: str
@@ -169,6 +183,16 @@ def padded_state(dims, nc, fill_fn) -> np.ndarray:
     I, J, K = np.meshgrid(np.arange(1, ni + 1), np.arange(1, nj + 1), np.arange(1, nk + 1), indexing="ij")
     a[:, 2:ni + 2, 2:nj + 2, 2:nk + 2] = fill_fn(I, J, K)
     return np.ascontiguousarray(a.ravel(order="F"))
+
+
+def padded_fields(dims, fields) -> np.ndarray:
+    """padded_state from the nc interior fields (each (ni, nj, nk), best laid out
+    in Fortran order, or scalars), written straight into the Fortran layout."""
+    ni, nj, nk = dims
+    a = np.full((nk + 4, nj + 4, ni + 4, len(fields)), np.nan)   # the Fortran layout, read backwards
+    for v, f in enumerate(fields):
+        a[2:nk + 2, 2:nj + 2, 2:ni + 2, v] = np.asarray(f).T
+    return a.ravel()
 
 
 def gas_arrays(prim5: np.ndarray, R, gam, k, mu) -> dict:
@@ -408,11 +432,36 @@ def case_crossing_jets(n: int = 1024, steps: int = 500, seed: int = 20261204, mo
 
 
 # ---------------------------------------------------------------- driving
+def _made_ahead(blocks, lazy, workers: int):
+    """The LazyBlocks `lazy` (1-based) made in order, up to `workers` of them
+    being made at a time on threads (numpy releases the interpreter lock in its
+    array loops), so the host holds only the blocks in flight."""
+    import concurrent.futures as cf
+    if not lazy:
+        return iter(())
+
+    def gen():
+        with cf.ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+            pending = []
+            nxt = 0
+            while nxt < len(lazy) and len(pending) < max(1, workers):
+                pending.append(ex.submit(blocks[lazy[nxt] - 1].make))
+                nxt += 1
+            while pending:
+                blk = pending.pop(0).result()
+                if nxt < len(lazy):
+                    pending.append(ex.submit(blocks[lazy[nxt] - 1].make))
+                    nxt += 1
+                yield blk
+                del blk
+    return gen()
+
+
 class Runner:
     """Feeds a Case to a library with the ice_gpu_ C-ABI (prefix 'ice_gpu_') or its
     CPU restatement (prefix 'orc_'), and reads states back."""
 
-    def __init__(self, lib, case: Case, prefix: str = "ice_gpu_", device: int = 0):
+    def __init__(self, lib, case: Case, prefix: str = "ice_gpu_", device: int = 0, workers: int = 4):
         import ctypes as C
         self.lib, self.case, self.pre = lib, case, prefix
         self.C = C
@@ -423,16 +472,20 @@ class Runner:
         self.ctx = ctx
         self.check(rc, "create")
         for b, B in enumerate(case.blocks, start=1):
-            g = B.geo
-            self.check(self.f("set_geometry")(ctx, b, g["vol"], g["dl"], g["M"], g["face_i"], g["face_j"],
-                                              g["face_k"]), "set_geometry")
+            if not isinstance(B, LazyBlock):
+                self.put_geometry(b, B)
         self.check(self.f("set_boundaries")(ctx, case.reci.shape[0], np.ascontiguousarray(case.reci),
                                             np.ascontiguousarray(case.recd), case.donor.shape[0],
                                             np.ascontiguousarray(case.donor.reshape(-1)),
                                             np.ascontiguousarray(case.weight)), "set_boundaries")
         for p, tab in case.meta.get("tables", {}).items():
             self.check(self.f("set_properties")(ctx, p, tab.t_min, tab.n, tab.cols, tab.flags), "set_properties")
+        lazy = [b for b, B in enumerate(case.blocks, start=1) if isinstance(B, LazyBlock)]
+        made = _made_ahead(case.blocks, lazy, workers)
         for b, B in enumerate(case.blocks, start=1):
+            if isinstance(B, LazyBlock):
+                B = next(made)
+                self.put_geometry(b, B)
             for p in range(1, case.cfg.ngroups + 1):
                 self.check(self.f("put_state")(ctx, b, p, B.prim[p - 1]), "put_state")
             if B.gas is not None and case.cfg.coupled:
@@ -440,6 +493,11 @@ class Runner:
                 self.check(self.f("put_gas")(ctx, b, gg["prim"], gg["R"], gg["gam"], gg["k"], gg["mu"]), "put_gas")
             if B.yinf is not None and case.cfg.var_yinf:
                 self.check(self.f("put_gas_yinf")(ctx, b, B.yinf), "put_gas_yinf")
+
+    def put_geometry(self, b: int, B: Block):
+        g = B.geo
+        self.check(self.f("set_geometry")(self.ctx, b, g["vol"], g["dl"], g["M"], g["face_i"], g["face_j"],
+                                          g["face_k"]), "set_geometry")
 
     def f(self, name):
         return getattr(self.lib, self.pre + name)
@@ -635,24 +693,7 @@ def case_multiblock(nblocks: int = 64, emin: int = 16, emax: int = 128, seed: in
         partner_face[(b, fb)] = (a, fa)
     tb = TableBuilder(ngroups)
     for b, dd in enumerate(dims):
-        def codes(f, i, j, k, b=b):
-            return np.array([101 if (b, int(ff)) in partner_face else 400 for ff in f], np.int32)
-
-        def ints(ri, fijk, b=b, dd=dd):
-            for f in range(1, 7):
-                if (b, f) not in partner_face:
-                    continue
-                ob, of = partner_face[(b, f)]
-                m = fijk[:, 0] == f
-                d = (f - 1) // 2
-                src = fijk[m, 1:4].copy()
-                # the cell across the face: same tangential indices, first interior cell of the other block
-                src[:, d] = 1 if of % 2 == 1 else dims[ob][d]
-                ri[m, R_BS] = ob + 1
-                ri[m, R_IS], ri[m, R_JS], ri[m, R_KS] = src[:, 0], src[:, 1], src[:, 2]
-                ri[m, R_FS] = of
-
-        tb.add_block(b + 1, dd, codes, ints_fn=ints)
+        tb.add_block(b + 1, dd, tree_codes(partner_face, b), ints_fn=tree_ints(partner_face, dims, b))
     reci, recd, donor, weight = tb.arrays()
     cfg = make_config(ngroups=ngroups, nrk=3, time_accurate=True, ndir=3, limiter=limiter, coupled=True,
                       drag_select=4, heat_select=8, cfl=0.5, mode=mode)
@@ -748,7 +789,7 @@ def case_evaporation(n: int = 256, steps: int = 50, seed: int = 20261206, mode: 
 # ---------------------------------------------------------------- chimera (overset) pair
 def case_chimera(n: int = 12, na: int = 16, shift: float = 3.7, closure: int = IG, ngroups: int = 1,
                  irs: bool = False, evap: int = 0, steps: int = 10, seed: int = 20261207, mode: int = 0,
-                 limiter: int = 3, connect: bool = False) -> Case:
+                 limiter: int = 3, connect: bool = False, closed_form: bool = False) -> Case:
     """Two overlapping blocks joined by chimera (102) records: A is na x n x n cells,
     B (na x n x n) starts `shift` cells before A's upper i-end, so that the two
     ghost layers of B's face 1 lie inside A and those of A's face 2 inside B.  Each
@@ -758,7 +799,8 @@ def case_chimera(n: int = 12, na: int = 16, shift: float = 3.7, closure: int = I
     moves with +x velocity through the overlap; the gas is a smooth random field.
     `closure` IG or AG, `ngroups` families, IRS (steady) and evaporation on demand.
     With shift = 0 the blocks abut; `connect` then joins them by face-matching
-    connections (101) instead, the same interface by another route."""
+    connections (101) instead, the same interface by another route.  closed_form:
+    the blocks' metrics in closed form (geometry.uniform_metrics), for large pairs."""
     rng = np.random.default_rng(seed)
     dx = 1e-3
     dims = (na, n, n)
@@ -769,8 +811,11 @@ def case_chimera(n: int = 12, na: int = 16, shift: float = 3.7, closure: int = I
     nc = NCOND[closure]
     ext = (x0[1] + na * dx)
     for b in range(2):
-        xs = [x0[b] + np.arange(na + 1) * dx, np.arange(n + 1) * dx, np.arange(n + 1) * dx]
-        geo = geometry.metrics(geometry.cartesian_nodes(*xs))
+        if closed_form:
+            geo = geometry.uniform_metrics(dims, dx)
+        else:
+            xs = [x0[b] + np.arange(na + 1) * dx, np.arange(n + 1) * dx, np.arange(n + 1) * dx]
+            geo = geometry.metrics(geometry.cartesian_nodes(*xs))
         xc = [x0[b] + (np.arange(na) + 0.5) * dx, (np.arange(n) + 0.5) * dx, (np.arange(n) + 0.5) * dx]
         X, Y, Z = np.meshgrid(xc[0] / ext, xc[1] / (n * dx), xc[2] / (n * dx), indexing="ij")
         g_rng = np.random.default_rng(gseed)
@@ -902,19 +947,57 @@ def case_from_inputs(name: str, ini: dict, nodes: List[np.ndarray], ic: List[np.
 
 
 # ---------------------------------------------------------------- combined configuration
+# The seed of the full combined configuration's tree: the first seed from 20261208
+# up whose 88-block tree (edges log-uniform on [32, 192]) holds 140-160 M cells.
+# Such trees are rare: the median tree of this generator holds 64 M cells, and 1
+# seed in 400 reaches 140 M; this one holds 142.6 M.
+COMBINED_FULL_SEED = 20261309
+
+
+def tree_codes(partner_face, b):
+    """code_fn of block b of a connection tree: 101 on joined faces, 400 elsewhere."""
+    joined = np.array([(b, f) in partner_face for f in range(0, 7)])
+
+    def codes(f, i, j, k):
+        return np.where(joined[np.asarray(f)], 101, 400).astype(np.int32)
+    return codes
+
+
+def tree_ints(partner_face, dims, b):
+    """ints_fn of block b of a connection tree: the source cell across each joined face."""
+    def ints(ri, fijk):
+        for f in range(1, 7):
+            if (b, f) not in partner_face:
+                continue
+            ob, of = partner_face[(b, f)]
+            m = fijk[:, 0] == f
+            d = (f - 1) // 2
+            src = fijk[m, 1:4].copy()
+            # the cell across the face: same tangential indices, first interior cell of the other block
+            src[:, d] = 1 if of % 2 == 1 else dims[ob][d]
+            ri[m, R_BS] = ob + 1
+            ri[m, R_IS], ri[m, R_JS], ri[m, R_KS] = src[:, 0], src[:, 1], src[:, 2]
+            ri[m, R_FS] = of
+    return ints
+
+
 def case_combined(ntree: int = 88, npairs: int = 4, emin: int = 32, emax: int = 192, pair_n: int = 24,
-                  seed: int = 20261208, steps: int = 10, mode: int = 0, dx: float = 1e-4) -> Case:
+                  seed: int = 20261208, steps: int = 10, mode: int = 0, dx: float = 1e-4, lazy: bool = False,
+                  pair_closed_form: bool = False) -> Case:
     """Combined: a seeded random tree of `ntree` face-matching blocks (edges
     log-uniform on [emin, emax], connections 101) plus `npairs` chimera overset
-    block pairs (102 donor lists with linear weights, as case_chimera), AG closure
-    with 2 families (HLLE, MUSCL van Leer), full interphase physics: drag
-    Schiller-Naumann, Ranz-Marshall heat, grey-body radiation, CEM evaporation with
-    the Langmuir-Knudsen interface and Stefan blowing.  Gas fields as the
-    evaporation case: seeded smooth random T 300-2000 K, p 1-5 bar, far-field
-    vapour fraction 0-0.3, |u| <= 10 m/s.  Steady SSP-RK3 with implicit residual
-    smoothing (beta 0.5).  Uniform Cartesian cells of side dx (closed-form metrics).
-    ntree = 88, npairs = 4, emin = 32, emax = 192 is the full configuration (96
-    blocks, ~150 M cells); smaller arguments give the same physics at test sizes."""
+    block pairs (102 donor lists with linear weights, as case_chimera, blocks of
+    2 pair_n x pair_n x pair_n cells), AG closure with 2 families (HLLE, MUSCL van
+    Leer), full interphase physics: drag Schiller-Naumann, Ranz-Marshall heat,
+    grey-body radiation, CEM evaporation with the Langmuir-Knudsen interface and
+    Stefan blowing.  Gas fields as the evaporation case: seeded smooth random T
+    300-2000 K, p 1-5 bar, far-field vapour fraction 0-0.3, |u| <= 10 m/s.  Steady
+    SSP-RK3 with implicit residual smoothing (beta 0.5).  Uniform Cartesian cells
+    of side dx (closed-form metrics).  case_combined_full() is the full
+    configuration; smaller arguments give the same physics at test sizes.
+    lazy: the tree's blocks are LazyBlocks, made one at a time as a Runner uploads
+    them (the full configuration's arrays would hold some 70 GB of host memory).
+    pair_closed_form: the chimera pairs' metrics in closed form (large pairs)."""
     rng = np.random.default_rng(seed)
     dims, orig, joins = block_tree(ntree, emin, emax, rng)
     rho_mat, ng = 1000.0, 2
@@ -924,14 +1007,14 @@ def case_combined(ntree: int = 88, npairs: int = 4, emin: int = 32, emax: int = 
     fseed = int(rng.integers(1 << 30))
     R, gam = 287.05, 1.4
     medians = [20e-6, 60e-6]
-    blocks = []
-    for b, (dd, o) in enumerate(zip(dims, orig)):
+
+    def make_block(dd, o):
         geo = geometry.uniform_metrics(dd, dx)
         c = [((o[m] - lo[m]) + np.arange(dd[m]) + 0.5) / span[m] for m in range(3)]
         frng = np.random.default_rng(fseed)   # the same fields everywhere, sampled at the block's cells
 
-        def field(frng=frng, c=c):
-            f = geometry.smooth_field_axes(frng, c[0], c[1], c[2], modes=8, kmax=3)
+        def field():
+            f = geometry.smooth_field_product(frng, c[0], c[1], c[2], modes=8, kmax=3)
             return 0.5 + 0.5 * np.tanh(f / 2.0)   # in (0, 1)
 
         u, v, w = (2.0 * field() - 1.0 for _ in range(3))
@@ -941,42 +1024,27 @@ def case_combined(ntree: int = 88, npairs: int = 4, emin: int = 32, emax: int = 
         Td = 280.0 + 70.0 * field()
         rr = field()
         g5 = np.stack([p / (R * Tg), 10.0 * u / np.sqrt(3.0), 10.0 * v / np.sqrt(3.0), 10.0 * w / np.sqrt(3.0), Tg])
+        del u, v, w, p
         gas = gas_arrays(g5, R, gam, 0.026 * (Tg / 300.0) ** 0.7, 1.8e-5 * (Tg / 300.0) ** 0.7)
         prims = []
+        rho = 1e-2 * (0.7 + 0.6 * rr)
         for q in range(ng):
             r = np.clip(medians[q] * np.exp(0.6 * (2.0 * rr - 1.0) * 2.0), 1e-6, 100e-6)
+            num = rho / (rho_mat * 4.0 / 3.0 * PI * r ** 3)
+            prims.append(padded_fields(dd, [rho, 0.9 * g5[1], 0.9 * g5[2], 0.9 * g5[3], rho, 0.0, 0.0, rho, 0.0, rho,
+                                            Td, num]))
+        return Block(dd, geo, prims, gas, yinf=np.ascontiguousarray(yinf.ravel(order="F")))
 
-            def fill(I, J, K, r=r):
-                rho = 1e-2 * (0.7 + 0.6 * rr)
-                num = rho / (rho_mat * 4.0 / 3.0 * PI * r ** 3)
-                z = np.zeros(I.shape)
-                return np.stack([rho, 0.9 * g5[1], 0.9 * g5[2], 0.9 * g5[3], rho, z, z, rho, z, rho, Td, num])
-
-            prims.append(padded_state(dd, 12, fill))
-        blocks.append(Block(dd, geo, prims, gas, yinf=np.ascontiguousarray(yinf.ravel(order="F"))))
+    blocks = []
+    for dd, o in zip(dims, orig):
+        blocks.append(LazyBlock(dd, lambda dd=dd, o=o: make_block(dd, o)) if lazy else make_block(dd, o))
     partner_face = {}
     for a, fa, b, fb in joins:
         partner_face[(a, fa)] = (b, fb)
         partner_face[(b, fb)] = (a, fa)
     tb = TableBuilder(ng)
     for b, dd in enumerate(dims):
-        def codes(f, i, j, k, b=b):
-            return np.array([101 if (b, int(ff)) in partner_face else 400 for ff in f], np.int32)
-
-        def ints(ri, fijk, b=b):
-            for f in range(1, 7):
-                if (b, f) not in partner_face:
-                    continue
-                ob, of = partner_face[(b, f)]
-                m = fijk[:, 0] == f
-                d = (f - 1) // 2
-                src = fijk[m, 1:4].copy()
-                src[:, d] = 1 if of % 2 == 1 else dims[ob][d]
-                ri[m, R_BS] = ob + 1
-                ri[m, R_IS], ri[m, R_JS], ri[m, R_KS] = src[:, 0], src[:, 1], src[:, 2]
-                ri[m, R_FS] = of
-
-        tb.add_block(b + 1, dd, codes, ints_fn=ints)
+        tb.add_block(b + 1, dd, tree_codes(partner_face, b), ints_fn=tree_ints(partner_face, dims, b))
     reci, recd, donor, weight = tb.arrays()
     # the chimera pairs, appended block by block after the tree
     ri_all, rd_all, dn_all, w_all = [reci], [recd], [donor], [weight]
@@ -984,7 +1052,7 @@ def case_combined(ntree: int = 88, npairs: int = 4, emin: int = 32, emax: int = 
     nd = donor.shape[0]
     for q in range(npairs):
         pc = case_chimera(n=pair_n, na=2 * pair_n, shift=3.7, closure=AG, ngroups=ng, irs=True, evap=2,
-                          seed=seed + 1 + q, mode=mode)
+                          seed=seed + 1 + q, mode=mode, closed_form=pair_closed_form)
         ri = pc.reci.copy()
         ri[:, R_B] += nb
         has = ri[:, R_BS] > 0
@@ -1011,3 +1079,12 @@ def case_combined(ntree: int = 88, npairs: int = 4, emin: int = 32, emax: int = 
                         ep=water_ep(alphae=0.04)) for _ in range(ng)]
     return Case("combined", cfg, fams, blocks, reci, recd, donor, weight, steps,
                 meta={"joins": joins, "tree_blocks": ntree, "pairs": npairs})
+
+
+def case_combined_full(mode: int = 0, steps: int = 10, lazy: bool = True) -> Case:
+    """The full combined configuration: 96 blocks, 88 in the connection tree (edges
+    log-uniform on [32, 192]) and 4 chimera pairs of 128 x 64 x 64 blocks, 146.8 M
+    cells in all, AG x 2 families with the full interphase physics, steady SSP-RK3
+    with IRS."""
+    return case_combined(ntree=88, npairs=4, emin=32, emax=192, pair_n=64, seed=COMBINED_FULL_SEED, steps=steps,
+                         mode=mode, lazy=lazy, pair_closed_form=True)

@@ -26,6 +26,7 @@ struct FamDev {
   double* buf[3] = {nullptr, nullptr, nullptr};  // three state buffers, roles rotated per stage
   int cur = 0, old = 0;
   double *beta = nullptr, *dt = nullptr, *tau = nullptr, *src = nullptr, *R = nullptr, *T = nullptr;
+  double* imports = nullptr;  // rates in force a coupler set (add_births), [nc][N], or null
   double *bsum = nullptr, *connF = nullptr;
   uint8_t* inj = nullptr;
   int* bidx = nullptr;
@@ -88,6 +89,7 @@ struct ice_gpu_ctx {
   double* d_weight = nullptr;
   std::vector<std::vector<int>> bfirst_h, brec_h;  // per family, 1-based (collect_boundary_cells)
   DevStatus* d_st = nullptr;
+  int* h_flags = nullptr;    // pinned mirror of the status's bad_state and bad_recon
   int iter = 0;
   int keep_source = 0;
   int kernel_timing = 0;
@@ -263,12 +265,14 @@ int build_lists(ice_gpu_ctx* c) {
       if ((R.v[ICE_R_TYPE] == 101 || R.v[ICE_R_TYPE] == 201) && R.owner) own.push_back(n0);
     }
     // records of each cell of each block, cells in k, j, i order, records by face (stable)
+    std::vector<std::vector<int>> of_blk(c->blk.size());  // the family's records of each block, in table order
+    for (int n0 : grp) of_blk[c->rec[n0].v[ICE_R_B] - 1].push_back(n0);
     for (size_t b = 0; b < c->blk.size(); ++b) {
       const BlkDev& B = c->blk[b];
       std::vector<std::pair<long long, int>> at;  // (cell key, record)
-      for (int n0 : grp) {
+      for (int n0 : of_blk[b]) {
         const int* v = c->rec[n0].v;
-        if (v[ICE_R_B] != (int)b + 1 || v[ICE_R_TYPE] == 0) continue;
+        if (v[ICE_R_TYPE] == 0) continue;
         long long key = (long long)(v[ICE_R_I] - 1) + (long long)B.ni * ((v[ICE_R_J] - 1) + (long long)B.nj * (v[ICE_R_K] - 1));
         at.push_back({key, n0});
       }
@@ -365,6 +369,7 @@ FamArgs fam_args(ice_gpu_ctx* c, int p) {
   a.cur = F.buf[F.cur]; a.old = F.buf[F.old];
   a.prim_w = F.buf[F.cur];
   a.beta = F.beta; a.dt = F.dt; a.tau = F.tau; a.inj = F.inj; a.src = F.src; a.R = F.R; a.T = F.T;
+  a.imports = F.imports;
   a.bsum = F.bsum; a.bidx = F.bidx; a.connF = F.connF;
   a.rec = c->d_rec; a.donor = c->d_donor; a.weight = c->d_weight;
   a.bfirst = F.bfirst; a.brec = F.brec; a.bcell_g = F.bcell_g; a.nbcell = F.nbcell;
@@ -377,7 +382,8 @@ FamArgs fam_args(ice_gpu_ctx* c, int p) {
   a.coupled = c->cfg.coupled; a.drag_sel = c->cfg.drag_select; a.heat_sel = c->cfg.heat_select;
   a.blow_sel = c->cfg.blow_select;
   a.nfrozen = fd.nfrozen;
-  for (int m = 0; m < 4; ++m) a.frozen[m] = fd.frozen[m];
+  static_assert(kNFrozenMax == ICE_NFROZEN_MAX, "frozen slots");
+  for (int m = 0; m < ICE_NFROZEN_MAX; ++m) a.frozen[m] = fd.frozen[m];
   a.keep_source = c->keep_source; a.var_yinf = c->cfg.var_yinf; a.iter = c->iter;
   a.cfl_ramp_iter = c->cfg.cfl_ramp_iter; a.ndir = c->cfg.ndir; a.fam = p;
   a.cfl = c->cfg.cfl; a.dt_max = c->cfg.dt_max; a.tau_factor = c->cfg.tau_factor; a.irs_beta = c->cfg.irs_beta;
@@ -666,6 +672,7 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
     }
   }
   if (dalloc(c, &c->d_st, 1)) return 1;
+  HIPCHK(hipHostMalloc((void**)&c->h_flags, 2 * sizeof(int), hipHostMallocDefault));
   DevStatus s0;
   std::memset(&s0, 0, sizeof s0);
   HIPCHK(hipMemcpy(c->d_st, &s0, sizeof s0, hipMemcpyHostToDevice));
@@ -825,10 +832,14 @@ static int state_copy_block(ice_gpu_ctx* c, int b, int p, const double* prim, do
     // Fortran (nc, cells) -> SoA slices, into every buffer
     for (long long g = 0; g < npad; ++g)
       for (int v = 0; v < F.nc; ++v) h[(size_t)v * npad + g] = prim[g * F.nc + v];
-    for (int m = 0; m < 3; ++m)
+    // one copy over the bus, then on the device into the other two buffers
+    for (int v = 0; v < F.nc; ++v)
+      HIPCHK(hipMemcpy(F.buf[0] + (long long)v * c->N + B.off, h.data() + (size_t)v * npad, npad * sizeof(double),
+                       hipMemcpyHostToDevice));
+    for (int m = 1; m < 3; ++m)
       for (int v = 0; v < F.nc; ++v)
-        HIPCHK(hipMemcpy(F.buf[m] + (long long)v * c->N + B.off, h.data() + (size_t)v * npad, npad * sizeof(double),
-                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(F.buf[m] + (long long)v * c->N + B.off, F.buf[0] + (long long)v * c->N + B.off,
+                         npad * sizeof(double), hipMemcpyDeviceToDevice));
   } else {
     for (int v = 0; v < F.nc; ++v)
       HIPCHK(hipMemcpy(h.data() + (size_t)v * npad, F.buf[F.cur] + (long long)v * c->N + B.off, npad * sizeof(double),
@@ -885,8 +896,24 @@ int ice_gpu_put_tau(ice_gpu_ctx* c, int b, int p, const double* tau) {
   return put_interior(c, b, c->fam[p - 1].tau, 1, tau, true);
 }
 
+int ice_gpu_get_tau(ice_gpu_ctx* c, int b, int p, double* tau) {
+  if (b < 1 || b > (int)c->blk.size() || p < 1 || p > c->cfg.ngroups) return fail(c, "[ICE::get_tau] no block/family");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const BlkDev& B = c->blk[b - 1];
+  long long npad = (long long)B.NI * B.NJ * B.NK;
+  std::vector<double> h((size_t)npad);
+  HIPCHK(hipMemcpy(h.data(), c->fam[p - 1].tau + B.off, npad * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 1; k <= B.nk; ++k)
+    for (int j = 1; j <= B.nj; ++j)
+      for (int i = 1; i <= B.ni; ++i)
+        tau[(i - 1) + (long long)B.ni * ((j - 1) + (long long)B.nj * (k - 1))] =
+            h[(size_t)((i + 1) + (long long)B.NI * ((j + 1) + (long long)B.NJ * (k + 1)))];
+  return 0;
+}
+
 int ice_gpu_set_keep_source(ice_gpu_ctx* c, int on) {
   c->keep_source = on == 2 ? 2 : (on ? 1 : 0);
+  batched_destroy(c);  // captured graphs hold the families' arguments
   for (auto& F : c->fam)
     if (on && !F.src) {
       if (dalloc(c, &F.src, (size_t)F.nc * c->N) || fill(c, F.src, (long long)F.nc * c->N, 0.0)) return 1;
@@ -906,6 +933,11 @@ int ice_gpu_step(ice_gpu_ctx* c, int nsteps, ice_status_t* st) {
   for (int s = 0; s < nsteps; ++s) {
     int rc = (c->cfg.mode == ICE_MODE_BATCHED && !any_table(c)) ? batched_step(c) : plain_step(c);
     if (rc) return rc;
+    // the reference stops in the stage whose state failed (Lib_Newstate.f90:143-147,
+    // Mod_Fluxes.f90:238-241): no step follows the one that set a fatal flag
+    HIPCHK(hipMemcpyAsync(c->h_flags, &c->d_st->bad_state, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->h_flags[0] || c->h_flags[1]) break;
     if (c->retrain_every > 0 && c->iter % c->retrain_every == 0 && retrain_networks(c)) return 1;
     if (res_stop) {
       // Mod_Explicit.f90:196-225: a steady run ends when its density residual has
@@ -1065,6 +1097,10 @@ int ice_gpu_set_tiles(ice_gpu_ctx* c, int tx, int ty, int tz) {
     if (bad)
       return fail(c, "[ICE::set_tiles] a %d x %d x %d tile has more cells or faces than the %d threads of a workgroup",
                   tx, ty, tz, kThreads);
+    for (int d = 0; d < 3; ++d)
+      if (n[d] > 1 && t[0] * t[1] * t[2] / t[d] * (t[d] + 4) > kMaxStageBox)
+        return fail(c, "[ICE::set_tiles] a %d x %d x %d tile's stencil box along direction %d holds more than %d cells",
+                    tx, ty, tz, d + 1, kMaxStageBox);
   }
   c->tile[0] = tx; c->tile[1] = ty; c->tile[2] = tz;
   batched_destroy(c);  // the batched plan's work items are tiles
@@ -1229,6 +1265,153 @@ int ice_gpu_set_kernel_timing(ice_gpu_ctx* c, int on) {
   return 0;
 }
 
+// compute_ghost and fill_second_ghost (Lib_Ghost.f90:29-72, 270-308) of every
+// family on its current state, family by family as the stages fill them
+int ice_gpu_fill_ghosts(ice_gpu_ctx* c) {
+  if (!c) return 1;
+  if (c->rec.empty()) return fail(c, "[ICE::fill_ghosts] no boundary table: call ice_gpu_set_boundaries first");
+  for (int p = 0; p < c->cfg.ngroups; ++p) {
+    const FamDev& F = c->fam[p];
+    FamArgs a = fam_args(c, p);
+    int cl = c->fams[p].closure;
+    if (cl == ICE_MK) launch_ghosts_mk(c->stream, F.grp, F.ngrp, a);
+    else if (cl == ICE_IG) launch_ghosts_ig(c->stream, F.grp, F.ngrp, a);
+    else launch_ghosts_ag(c->stream, F.grp, F.ngrp, a);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// The rates in force on family p in block b (the imports of clear_import_rates /
+// add_birth_rate / add_absorption_rate, Mod_Exchange.f90:242-324), conservative
+// slots (nc, ni, nj, nk); every stage adds them over the cell's dt (add_births).
+// imports = NULL clears them.
+int ice_gpu_put_imports(ice_gpu_ctx* c, int b, int p, const double* imports) {
+  if (!c) return 1;
+  if (b < 1 || b > (int)c->blk.size() || p < 1 || p > c->cfg.ngroups)
+    return fail(c, "[ICE::put_imports] no block %d / family %d", b, p);
+  FamDev& F = c->fam[p - 1];
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!F.imports) {
+    if (dalloc(c, &F.imports, (size_t)F.nc * c->N) || fill(c, F.imports, (long long)F.nc * c->N, 0.0)) return 1;
+    batched_destroy(c);  // captured graphs hold the families' arguments
+  }
+  if (imports) {
+    if (put_interior(c, b, F.imports, F.nc, imports, true)) return 1;
+  } else {
+    const BlkDev& B = c->blk[b - 1];
+    for (int v = 0; v < F.nc; ++v)
+      if (fill(c, F.imports + (long long)v * c->N + B.off, (long long)B.NI * B.NJ * B.NK, 0.0)) return 1;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// add_birth (Mod_Exchange.f90:283-300) of n packets of family p: packet m, born in
+// cell (b, i, j, k) = cells[4m .. 4m+3] (1-based), is pk[6m .. 6m+5] = [mass,
+// momentum (3), energy, number] per unit volume (birth_packet), added to the cell's
+// conservative state; the packets of one cell in their order.
+int ice_gpu_add_births(ice_gpu_ctx* c, int p, int n, const int32_t* cells, const double* pk) {
+  if (!c) return 1;
+  if (p < 1 || p > c->cfg.ngroups) return fail(c, "[ICE::add_births] no family %d", p);
+  if (n <= 0) return 0;
+  std::vector<std::pair<long long, int>> at((size_t)n);
+  for (int m = 0; m < n; ++m) {
+    const int* x = cells + 4 * (size_t)m;
+    if (x[0] < 1 || x[0] > (int)c->blk.size()) return fail(c, "[ICE::add_births] packet %d: no block %d", m + 1, x[0]);
+    const BlkDev& B = c->blk[x[0] - 1];
+    if (x[1] < 1 || x[1] > B.ni || x[2] < 1 || x[2] > B.nj || x[3] < 1 || x[3] > B.nk)
+      return fail(c, "[ICE::add_births] packet %d: cell %d %d %d is not in block %d", m + 1, x[1], x[2], x[3], x[0]);
+    at[m] = {padded_of(B, x[1], x[2], x[3]), m};
+  }
+  std::stable_sort(at.begin(), at.end(),
+                   [](const std::pair<long long, int>& u, const std::pair<long long, int>& v) { return u.first < v.first; });
+  std::vector<long long> cell;
+  std::vector<int> first;
+  std::vector<double> pks((size_t)n * 6);
+  for (size_t m = 0; m < at.size(); ++m) {
+    if (m == 0 || at[m].first != at[m - 1].first) {
+      cell.push_back(at[m].first);
+      first.push_back((int)m);
+    }
+    std::memcpy(&pks[m * 6], pk + (size_t)at[m].second * 6, 6 * sizeof(double));
+  }
+  first.push_back(n);
+  long long* d_cell = nullptr;
+  int* d_first = nullptr;
+  double* d_pk = nullptr;
+  int rc = dalloc(c, &d_cell, cell.size()) || dalloc(c, &d_first, first.size()) || dalloc(c, &d_pk, pks.size());
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipMemcpy(d_cell, cell.data(), cell.size() * sizeof(long long), hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess) e = hipMemcpy(d_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess) e = hipMemcpy(d_pk, pks.data(), pks.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (!rc && e == hipSuccess) {
+    FamArgs a = fam_args(c, p - 1);
+    int cl = c->fams[p - 1].closure, nc = (int)cell.size();
+    if (cl == ICE_MK) launch_births_mk(c->stream, a, d_cell, d_first, d_pk, nc);
+    else if (cl == ICE_IG) launch_births_ig(c->stream, a, d_cell, d_first, d_pk, nc);
+    else launch_births_ag(c->stream, a, d_cell, d_first, d_pk, nc);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (d_cell) (void)hipFree(d_cell);
+  if (d_first) (void)hipFree(d_first);
+  if (d_pk) (void)hipFree(d_pk);
+  if (rc) return 1;
+  if (e != hipSuccess) return fail(c, "[ICE::add_births] %s", hipGetErrorString(e));
+  return 0;
+}
+
+int ice_gpu_set_mode(ice_gpu_ctx* c, int mode) {
+  if (!c) return 1;
+  if (mode != ICE_MODE_PLAIN && mode != ICE_MODE_BATCHED)
+    return fail(c, "[ICE::set_mode] mode is ICE_MODE_PLAIN (0) or ICE_MODE_BATCHED (1)");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->cfg.mode = mode;
+  return 0;
+}
+
+int ice_gpu_set_surrogate(ice_gpu_ctx* c, int p, int mode) {
+  if (!c) return 1;
+  if (p < 1 || p > c->cfg.ngroups) return fail(c, "[ICE::set_surrogate] no family %d", p);
+  if (mode < 0 || mode > 2) return fail(c, "[ICE::set_surrogate] mode is 0 (plain), 1 (network) or 2 (no network)");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->fams[p - 1].surrogate = mode;
+  batched_destroy(c);  // captured graphs hold the families' arguments
+  return 0;
+}
+
+// Restart (Wrap_Setup.f90:158-214): grid%iter, grid%time and res_reference as the
+// solution read stored them (IO_Solution.f90:20-30), so that the CFL ramp
+// (Mod_dt.f90:83-85), the time of a time-accurate run and the unit of
+// res-threshold go on as in the uninterrupted run.
+int ice_gpu_set_counters(ice_gpu_ctx* c, int32_t iter, double time, double res_reference) {
+  if (!c) return 1;
+  if (iter < 0 || !std::isfinite(time) || !(res_reference >= 0.0))
+    return fail(c, "[ICE::set_counters] iter >= 0, a finite time and res_reference >= 0");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  DevStatus s;
+  HIPCHK(hipMemcpy(&s, c->d_st, sizeof s, hipMemcpyDeviceToHost));
+  s.iter = iter;
+  s.time = time;
+  HIPCHK(hipMemcpy(c->d_st, &s, sizeof s, hipMemcpyHostToDevice));
+  c->iter = iter;
+  c->res_reference = res_reference;
+  return 0;
+}
+
+int ice_gpu_get_counters(ice_gpu_ctx* c, int32_t* iter, double* time, double* res_reference) {
+  if (!c) return 1;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  DevStatus s;
+  HIPCHK(hipMemcpy(&s, c->d_st, sizeof s, hipMemcpyDeviceToHost));
+  if (iter) *iter = c->iter;
+  if (time) *time = s.time;
+  if (res_reference) *res_reference = c->res_reference;
+  return 0;
+}
+
 // Two contexts one level apart (Check_Multigrid, Lib_Multigrid.f90:12-35): same
 // device, families and blocks, each coarse block half the fine one per direction
 int level_pair(ice_gpu_ctx* f, ice_gpu_ctx* c) {
@@ -1379,7 +1562,7 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
   F(c->yinf); F(c->d_rec); F(c->d_donor); F(c->d_weight); F(c->d_st);
   for (auto& f : c->fam) {
     for (int m = 0; m < 3; ++m) F(f.buf[m]);
-    F(f.beta); F(f.dt); F(f.tau); F(f.src); F(f.R); F(f.T); F(f.bsum); F(f.connF); F(f.inj); F(f.bidx);
+    F(f.beta); F(f.dt); F(f.tau); F(f.src); F(f.R); F(f.T); F(f.bsum); F(f.connF); F(f.inj); F(f.bidx); F(f.imports);
     F(f.grp); F(f.own); F(f.bfirst); F(f.brec); F(f.bcell_g); F(f.prop);
   }
   for (auto& S : c->surr) {
@@ -1390,6 +1573,7 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
   }
   F(c->i400_lines); F(c->i400_n); F(c->res_part); F(c->res_sums);
   batched_destroy(c);
+  if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1622,7 +1806,7 @@ int batched_step(ice_gpu_ctx* c) {
     StepArgs A;
     if (prepare_step_args(c, A, G->d_args)) return 1;
     G->end_cur = A.end_cur;
-    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeGlobal));
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     launch_batched_step(c, A, false);
     hipError_t e = hipStreamEndCapture(c->stream, &G->graph);
     if (e != hipSuccess) return fail(c, "hipStreamEndCapture: %s", hipGetErrorString(e));

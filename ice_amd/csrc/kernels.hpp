@@ -11,6 +11,7 @@
 
 namespace ice {
 
+constexpr int kNFrozenMax = 6;  // frozen slots of a family: v, w and AG's P12, P13, P23 (Mod_Freeze.f90:117-128)
 constexpr int kGuide[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
 
 struct BlkDev {
@@ -65,6 +66,7 @@ struct FamArgs {
   double* tau;          // [N]
   uint8_t* inj;         // [N]
   double* src;          // [nc][N] kept source (optional)
+  const double* imports;  // [nc][N] rates in force a coupler set (births, absorptions), or null
   double* R;            // [nc][N] materialised residual (IRS)
   double* T;            // [nc][N] IRS buffer: the directions ping-pong between R and T
   double* bsum;         // [nbcell][nc] boundary-flux sums
@@ -83,7 +85,7 @@ struct FamArgs {
   DevStatus* st;
   MatDev mat;
   int limiter, nrk, srk, time_accurate, coupled, drag_sel, heat_sel, blow_sel;
-  int nfrozen, frozen[4];
+  int nfrozen, frozen[kNFrozenMax];
   int keep_source, last_stage, var_yinf, iter, cfl_ramp_iter, ndir, fam;
   double cfl, dt_max, tau_factor, irs_beta;
   // learned seed of the interface solve (mat.surrogate = 1) and its counters

@@ -104,8 +104,7 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
   }
   if (L.irs) {
     const double* res = launch_irs<NC>(a, L.ndir, L.irs_wg, L.irs_nwg, nb, L.stream);
-    for (int b = 0; b < nb; ++b)
-      hipLaunchKernelGGL(k_update<M>, dim3(grid_cells(L.blk[b].cells)), dim3(kThreads), 0, L.stream, a, b, res);
+    hipLaunchKernelGGL(k_update<M>, dim3((unsigned)L.irs_nwg), dim3(kThreads), 0, L.stream, a, res, L.irs_wg, nb);
   }
 }
 
@@ -114,6 +113,22 @@ template <int M, int S> void launch_sd_cp(const StageLaunch& L, const FamArgs& a
   else if (L.sd) launch_stage_kernels<M, S, true, false>(L, a);
   else if (L.coupled) launch_stage_kernels<M, S, false, true>(L, a);
   else launch_stage_kernels<M, S, false, false>(L, a);
+}
+
+// the two ghost layers of a family's current state (compute_ghost, fill_second_ghost)
+template <int M> void launch_ghost_kernels(hipStream_t stream, const int* grp, int ngrp, FamArgs a) {
+  a.list = grp;
+  a.nlist = ngrp;
+  if (!ngrp) return;
+  hipLaunchKernelGGL(k_ghost1<M>, dim3(grid_cells(ngrp)), dim3(kThreads), 0, stream, a);
+  hipLaunchKernelGGL(k_ghost2<M>, dim3(grid_cells(ngrp)), dim3(kThreads), 0, stream, a);
+}
+
+template <int M>
+void launch_births_kernel(hipStream_t stream, const FamArgs& a, const long long* cell, const int* first,
+                          const double* pk, int ncell) {
+  if (ncell > 0)
+    hipLaunchKernelGGL(k_add_births<M>, dim3(grid_cells(ncell)), dim3(kThreads), 0, stream, a, cell, first, pk, ncell);
 }
 
 template <int M> void launch_dt_kernel(hipStream_t stream, const BlkDev* blk, int nb, const FamArgs& a) {
@@ -188,8 +203,7 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
     for (int f = 0; f < B.nfam; ++f) {
       FamArgs a = B.h_fams[B.fam_ids[f]];
       const double* res = launch_irs<NC>(a, B.ndir, B.irs_wg, B.irs_nwg, B.nb, B.stream);
-      for (int b = 0; b < B.nb; ++b)
-        hipLaunchKernelGGL(k_update<M>, dim3(grid_cells(B.blk[b].cells)), blk, 0, B.stream, a, b, res);
+      hipLaunchKernelGGL(k_update<M>, dim3((unsigned)B.irs_nwg), blk, 0, B.stream, a, res, B.irs_wg, B.nb);
     }
     return;
   }
@@ -240,6 +254,12 @@ int persistent_wgs_ig(int cus);
 int persistent_wgs_ag(int cus);
 
 // defined in stage_mk.hip, stage_ig.hip, stage_ag.hip
+void launch_ghosts_mk(hipStream_t s, const int* grp, int ngrp, const FamArgs& a);
+void launch_ghosts_ig(hipStream_t s, const int* grp, int ngrp, const FamArgs& a);
+void launch_ghosts_ag(hipStream_t s, const int* grp, int ngrp, const FamArgs& a);
+void launch_births_mk(hipStream_t s, const FamArgs& a, const long long* cell, const int* first, const double* pk, int n);
+void launch_births_ig(hipStream_t s, const FamArgs& a, const long long* cell, const int* first, const double* pk, int n);
+void launch_births_ag(hipStream_t s, const FamArgs& a, const long long* cell, const int* first, const double* pk, int n);
 void launch_stage_mk(const StageLaunch& L, const FamArgs& a);
 void launch_stage_ig(const StageLaunch& L, const FamArgs& a);
 void launch_stage_ag(const StageLaunch& L, const FamArgs& a);

@@ -7,6 +7,13 @@ void launch_stage_ig(const StageLaunch& L, const FamArgs& a) {
   else launch_sd_cp<1, 1>(L, a);
 }
 void launch_dt_ig(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<1>(s, blk, nb, a); }
+void launch_ghosts_ig(hipStream_t s, const int* grp, int ngrp, const FamArgs& a) {
+  launch_ghost_kernels<1>(s, grp, ngrp, a);
+}
+void launch_births_ig(hipStream_t s, const FamArgs& a, const long long* cell, const int* first, const double* pk,
+                      int n) {
+  launch_births_kernel<1>(s, a, cell, first, pk, n);
+}
 void launch_batched_ig(const BatchLaunch& B) {
   if (B.riemann == 2) launch_batched_sd_cp<1, 2>(B);
   else launch_batched_sd_cp<1, 1>(B);

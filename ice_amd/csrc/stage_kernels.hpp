@@ -517,6 +517,41 @@ template <int NC> ICE_DEV void keep_stage_source(const FamArgs& a, long long g, 
   }
 }
 
+// add_births (Lib_Residual.f90:164-210): the rates a coupler set, over the cell's dt
+template <int NC> ICE_DEV void add_imports(const FamArgs& a, long long g, double dt, double* r) {
+  if (!a.imports) return;
+#pragma unroll
+  for (int v = 0; v < NC; ++v) r[v] = r[v] + a.imports[(long long)v * a.N + g] * dt;
+}
+
+// add_birth (Mod_Exchange.f90:283-300): packets pk = [mass, momentum (3), energy,
+// number] added at once to the conservative state of their cells in the current
+// buffer; cell c (padded index cell[c]) takes packets first[c] .. first[c+1]-1 in
+// their order, all by one thread
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_add_births(FamArgs a, const long long* __restrict__ cell,
+                                                         const int* __restrict__ first, const double* __restrict__ pk,
+                                                         int ncell) {
+  constexpr int NC = Closure<M>::NC;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const long long g = cell[c];
+  double q[NC], u[NC];
+  load_cell<NC>(a.cur, a.N, g, q);
+  for (int n = first[c]; n < first[c + 1]; ++n) {
+    const double* k = pk + (long long)n * 6;
+    prim2cons<M>(q, u, a.mat);
+    u[0] = u[0] + k[0];
+    u[1] = u[1] + k[1];
+    u[2] = u[2] + k[2];
+    u[3] = u[3] + k[3];
+    u[NC - 2] = u[NC - 2] + k[4];
+    u[NC - 1] = u[NC - 1] + k[5];
+    cons2prim<M>(u, q, a.mat);
+  }
+  store_cell<NC>(a.prim_w, a.N, g, q);
+}
+
 // ------------------------------------------------------------------ update
 // state_update_ (Lib_Newstate.f90:152-173) of one cell
 template <int M, bool TAB = true>
@@ -610,6 +645,7 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage(FamArg
   double dt = a.time_accurate ? a.st->dtglobal : a.dt[g];
 #pragma unroll
   for (int v = 0; v < NC; ++v) r[v] = (r[v] / vol + src[v]) * dt;
+  add_imports<NC>(a, g, dt, r);
   for (int f = 0; f < a.nfrozen; ++f) r[a.frozen[f] - 1] = 0.0;
   if constexpr (WRITE_R) {
     store_cell<NC>(a.R, a.N, g, r);
@@ -681,12 +717,16 @@ __global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int d, const double
   for (int v = 0; v < NC; ++v) dst[(long long)v * a.N + g] = r[v];
 }
 
-// the update of block b from the smoothed residual res
+// the update of every block of the family from the smoothed residual res, in one
+// launch (workgroups wg[b] .. wg[b+1]-1 take block b, as in k_irs)
 template <int M>
-__global__ void __launch_bounds__(kThreads) k_update(FamArgs a, int b, const double* __restrict__ res) {
+__global__ void __launch_bounds__(kThreads) k_update(FamArgs a, const double* __restrict__ res,
+                                                     const int* __restrict__ wg, int nb) {
   constexpr int NC = Closure<M>::NC;
+  const int w = (int)blockIdx.x;
+  const int b = block_of_workgroup(wg, nb, w);
   const BlkDev& B = a.blk[b];
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long t = (long long)(w - wg[b]) * blockDim.x + threadIdx.x;
   if (t >= B.cells) return;
   int i, j, k;
   cell_of(B, t, i, j, k);
@@ -709,6 +749,43 @@ struct Tile {
 
 // slots of the face fluxes of one direction in LDS, structure of arrays
 constexpr int kMaxTileFaces = kThreads;
+// cells of the stencil box of one direction of a tile: the tile's cells widened
+// by two along the direction, t_d + 4 by the other two extents
+constexpr int kMaxStageBox = 512;
+// doubles of the LDS stencil box of a closure (its nc slots and the cells'
+// lengths along the direction), shared with the learned seed's buffer, which is
+// used only after the faces
+template <int NC> constexpr int stage_lds_doubles() {
+  return (NC + 1) * kMaxStageBox > kMlpLds / 4 ? (NC + 1) * kMaxStageBox : kMlpLds / 4;
+}
+
+// face_acc (MODE 2) with the face's four stencil cells and their lengths along d
+// read from the tile's LDS box: box[v * nbox + c] holds slot v of box cell c, slot
+// NC the cell's length along d; c is the box cell of the face's lower cell, bstride
+// the box's stride along d.  The same operations as face_acc on the same values.
+template <int M, int S, bool SD, int LIM, bool TAB>
+ICE_DEV bool face_box(const FamArgs& a, long long g, long long stride, int d, int m, int n, const double* box,
+                      int nbox, int c, int bstride, double* r) {
+  constexpr int NC = Closure<M>::NC;
+  const long long N = a.N;
+  const double* len = box + NC * nbox;
+  double len0 = len[c - bstride], len1 = len[c], len2 = len[c + bstride], len3 = len[c + 2 * bstride];
+  double bf = SD ? dmin2(a.beta[g], a.beta[g + stride]) : 1.0;
+  if (m == 1 && a.inj[g - stride]) len0 = 0.0;
+  if (m == n - 1 && a.inj[g + 2 * stride]) len3 = 0.0;
+  const double* fa = a.fa + (long long)(d * 4) * N;
+  double nrm[3] = {fa[g], fa[N + g], fa[2 * N + g]};
+  double area = fa[3 * N + g];
+  double dl0 = 0.5 * (len0 + len1);
+  double dl1 = 0.5 * (len1 + len2);
+  double dl2 = 0.5 * (len2 + len3);
+  double dll = 0.5 * len1;
+  double dlr = 0.5 * len2;
+  double ql[NC], qr[NC];
+  bool ok = reconstruct_mem<M, LIM>(a.limiter, box + (c - bstride), bstride, nbox, dl0, dl1, dl2, dll, dlr, bf, ql, qr);
+  riemann_acc<M, S, 2>(ql, qr, nrm, area, r, mat_of<TAB>(a));
+  return ok;
+}
 
 // The stage with the interior faces shared through LDS (Mod_Fluxes.f90:421-521:
 // each face once, each cell its two faces per direction in the odd-then-even
@@ -718,7 +795,8 @@ constexpr int kMaxTileFaces = kThreads;
 // LDS buffer of face fluxes.
 // TAB = false: the launch serves only families without tabulated properties
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM, bool TAB = false>
-ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl, uint16_t* mlds) {
+ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl, double* box,
+                             uint16_t* mlds) {
   constexpr int NC = Closure<M>::NC;
   const BlkDev& Bref = a.blk[b];
   const BlkDev B = Bref;
@@ -750,15 +828,39 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
     int fx = T.tx + (d == 0), fy = T.ty + (d == 1), fz = T.tz + (d == 2);
     int nf = fx * fy * fz;
     long long stride = dir_stride(B, d);
+    // the stencil box: the tile's cells widened by two along d (offsets -2 .. t_d+1),
+    // each cell's nc slots and its length along d, loaded once from HBM/L2; cells
+    // past the block's ghost layers or its other extents are not loaded (no face
+    // reads them)
+    const int bx = T.tx + 4 * (d == 0), by = T.ty + 4 * (d == 1), bz = T.tz + 4 * (d == 2);
+    const int nbox = bx * by * bz;
+    const int x0 = ox - 2 * (d == 0), y0 = oy - 2 * (d == 1), z0 = oz - 2 * (d == 2);
+    const int xm = B.ni + 2 * (d == 0), ym = B.nj + 2 * (d == 1), zm = B.nk + 2 * (d == 2);
+    const double* dlen = a.dl + (long long)d * a.N;
+    for (int c = t; c < nbox; c += blockDim.x) {
+      int bi = c % bx, bj = (c / bx) % by, bk = c / (bx * by);
+      int i = x0 + bi, j = y0 + bj, k = z0 + bk;
+      if (i <= xm && j <= ym && k <= zm) {
+        long long gg = gidx(B, i, j, k);
+#pragma unroll
+        for (int v = 0; v < NC; ++v) box[v * nbox + c] = a.cur[(long long)v * a.N + gg];
+        box[NC * nbox + c] = dlen[gg];
+      }
+    }
+    const int bstride = d == 0 ? 1 : (d == 1 ? bx : bx * by);
+    __syncthreads();
     for (int f = t; f < nf; f += blockDim.x) {
-      int li = ox + f % fx - (d == 0);
-      int lj = oy + (f / fx) % fy - (d == 1);
-      int lk = oz + f / (fx * fy) - (d == 2);
+      int fi = f % fx, fj = (f / fx) % fy, fk = f / (fx * fy);
+      int li = ox + fi - (d == 0);
+      int lj = oy + fj - (d == 1);
+      int lk = oz + fk - (d == 2);
       int m = d == 0 ? li : (d == 1 ? lj : lk);
       bool inside = m >= 1 && m <= n - 1 && li >= 1 && lj >= 1 && lk >= 1 && li <= B.ni && lj <= B.nj && lk <= B.nk;
       if (inside) {
+        // the box cell of the face's lower cell: offset +1 along d from the face grid
+        int c = (fi + (d == 0)) + bx * ((fj + (d == 1)) + by * (fk + (d == 2)));
         double q[NC];
-        ok = face_acc<M, S, SD, 2, LIM, TAB>(a, gidx(B, li, lj, lk), stride, d, m, n, q) && ok;
+        ok = face_box<M, S, SD, LIM, TAB>(a, gidx(B, li, lj, lk), stride, d, m, n, box, nbox, c, bstride, q) && ok;
 #pragma unroll
         for (int v = 0; v < NC; ++v) fl[v * kMaxTileFaces + f] = q[v];
       }
@@ -809,6 +911,7 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
   double dt = a.time_accurate ? a.st->dtglobal : a.dt[g];
 #pragma unroll
   for (int v = 0; v < NC; ++v) r[v] = (r[v] / vol + src[v]) * dt;
+  add_imports<NC>(a, g, dt, r);
   for (int f = 0; f < a.nfrozen; ++f) r[a.frozen[f] - 1] = 0.0;
   if constexpr (WRITE_R) {
     store_cell<NC>(a.R, a.N, g, r);
@@ -833,8 +936,8 @@ template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_tiled(FamArgs a, int b, Tile T) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
-  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
-  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(a, b, T, xcd_tile(blockIdx.x, gridDim.x), fl, mlds);
+  __shared__ __attribute__((aligned(16))) double box[stage_lds_doubles<NC>()];
+  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(a, b, T, xcd_tile(blockIdx.x, gridDim.x), fl, box, (uint16_t*)box);
 }
 
 // ------------------------------------------------------------------ batched path
@@ -874,14 +977,14 @@ template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_persist(const FamArgs* fams, WorkQueues Q) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
-  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
+  __shared__ __attribute__((aligned(16))) double box[stage_lds_doubles<NC>()];
   __shared__ int slot;
   int q = blockIdx.x % Q.nq;
   for (;;) {
     int it = next_item(Q, q, &slot);
     if (it < 0) break;
     const WorkItem w = Q.items[it];
-    stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
+    stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, box, (uint16_t*)box);
   }
 }
 
@@ -894,10 +997,10 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_items(
                                                                                  const WorkItem* items) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
-  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
+  __shared__ __attribute__((aligned(16))) double box[stage_lds_doubles<NC>()];
   const int it = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x));
   const WorkItem w = items[it];
-  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
+  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, box, (uint16_t*)box);
 }
 
 // the sensor weights and the dt pass over the same queues, one cell per thread

@@ -8,6 +8,13 @@ void launch_stage_mk(const StageLaunch& L, const FamArgs& a) {
   else launch_sd_cp<0, 2>(L, a);
 }
 void launch_dt_mk(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<0>(s, blk, nb, a); }
+void launch_ghosts_mk(hipStream_t s, const int* grp, int ngrp, const FamArgs& a) {
+  launch_ghost_kernels<0>(s, grp, ngrp, a);
+}
+void launch_births_mk(hipStream_t s, const FamArgs& a, const long long* cell, const int* first, const double* pk,
+                      int n) {
+  launch_births_kernel<0>(s, a, cell, first, pk, n);
+}
 void launch_batched_mk(const BatchLaunch& B) {
   if (B.riemann == 0) launch_batched_sd_cp<0, 0>(B);
   else if (B.riemann == 1) launch_batched_sd_cp<0, 1>(B);

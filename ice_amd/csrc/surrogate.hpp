@@ -473,6 +473,7 @@ static int surr_train_impl(ice_gpu_ctx* c, int p, int epochs, int batch, double 
     sum += d;
     mx = std::max(mx, d);
   }
+  if (!S.on) batched_destroy(c);  // captured graphs hold the families' arguments (mlp_on)
   S.on = 1;
   S.batch = batch;
   S.lr = lr;
@@ -515,6 +516,7 @@ int ice_gpu_surrogate_set_params(ice_gpu_ctx* c, int p, const float* params) {
   HIPCHK(hipMemcpy(S.P, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
   if (surr_pack(c, S)) return 1;
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (!S.on) batched_destroy(c);  // captured graphs hold the families' arguments (mlp_on)
   S.on = 1;
   return 0;
 }
@@ -522,6 +524,7 @@ int ice_gpu_surrogate_set_params(ice_gpu_ctx* c, int p, const float* params) {
 int ice_gpu_set_lk_stats(ice_gpu_ctx* c, int on) {
   HIPCHK(hipStreamSynchronize(c->stream));
   c->lk_stats_on = on ? 1 : 0;
+  batched_destroy(c);  // captured graphs hold the families' arguments
   for (auto& S : c->surr) {
     if (!S.stats && dalloc(c, &S.stats, 4)) return 1;
     const unsigned long long z[4] = {0, 0, 0, 0};

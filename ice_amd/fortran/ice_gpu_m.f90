@@ -20,7 +20,7 @@ module ice_gpu_m
 
   !> ice_family_t
   type, bind(C), public :: ice_family_t
-    integer(c_int32_t) :: closure = ICE_IG, riemann = ICE_RUSANOV, nfrozen = 0, frozen(4) = 0
+    integer(c_int32_t) :: closure = ICE_IG, riemann = ICE_RUSANOV, nfrozen = 0, frozen(6) = 0
     integer(c_int32_t) :: evap_select = 0, intf_select = 0, surrogate = 0
     real(c_double)     :: rho_mat = 0, cs = 0, lv = 0, emiss = 0, ep(8) = 0
   end type
@@ -66,7 +66,9 @@ module ice_gpu_m
             ice_gpu_put_gas_yinf, ice_gpu_set_keep_source, ice_gpu_set_res_threshold, ice_gpu_res_stop, &
             ice_gpu_restrict, ice_gpu_prolong, ice_gpu_set_tiles, ice_gpu_set_dispatch, ice_gpu_set_kernel_timing, &
             ice_gpu_mlp_grad, ice_gpu_mlp_forward, ice_gpu_surrogate_train, ice_gpu_surrogate_retrain_every, &
-            ice_gpu_surrogate_params, ice_gpu_surrogate_set_params, ice_gpu_set_lk_stats, ice_gpu_lk_stats
+            ice_gpu_surrogate_params, ice_gpu_surrogate_set_params, ice_gpu_set_lk_stats, ice_gpu_lk_stats, &
+            ice_gpu_set_mode, ice_gpu_set_surrogate, ice_gpu_set_counters, ice_gpu_get_counters, &
+            ice_gpu_fill_ghosts, ice_gpu_put_imports, ice_gpu_add_births, ice_gpu_get_tau
   integer, parameter, public :: ICE_MLP_PARAMS = 6288
 
   interface
@@ -270,6 +272,50 @@ module ice_gpu_m
       type(c_ptr), value :: ctx
       integer(c_int), value :: p
       integer(c_int64_t) :: counts(4)
+    end function
+    integer(c_int) function ice_gpu_get_tau(ctx, b, p, tau) bind(C, name='ice_gpu_get_tau')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: b, p
+      real(c_double) :: tau(*)
+    end function
+    integer(c_int) function ice_gpu_fill_ghosts(ctx) bind(C, name='ice_gpu_fill_ghosts')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function ice_gpu_put_imports(ctx, b, p, imports) bind(C, name='ice_gpu_put_imports')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, imports
+      integer(c_int), value :: b, p
+    end function
+    integer(c_int) function ice_gpu_add_births(ctx, p, n, cells, pk) bind(C, name='ice_gpu_add_births')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: p, n
+      integer(c_int32_t) :: cells(*)
+      real(c_double) :: pk(*)
+    end function
+    integer(c_int) function ice_gpu_set_mode(ctx, mode) bind(C, name='ice_gpu_set_mode')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+    end function
+    integer(c_int) function ice_gpu_set_surrogate(ctx, p, mode) bind(C, name='ice_gpu_set_surrogate')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: p, mode
+    end function
+    integer(c_int) function ice_gpu_set_counters(ctx, iter, time, res_reference) bind(C, name='ice_gpu_set_counters')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: iter
+      real(c_double), value :: time, res_reference
+    end function
+    integer(c_int) function ice_gpu_get_counters(ctx, iter, time, res_reference) bind(C, name='ice_gpu_get_counters')
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: iter
+      real(c_double) :: time, res_reference
     end function
     integer(c_int) function ice_gpu_last_error(ctx, buf, len) bind(C, name='ice_gpu_last_error')
       import :: c_int, c_char, c_ptr

@@ -28,30 +28,40 @@ module ice_host_m
   !> ([ICE-Multigrid] levels and level<m>-iter, Register_Numerics.f90:90-115: up to
   !> MAXLEV levels here)
   integer, parameter, public :: MAXLEV = 4
-  integer, parameter :: NKEY = 27
+  integer, parameter :: NKEY = 43
   character(len=24), parameter :: key_sec(NKEY) = [character(len=24) :: &
     'ICE-Parameters', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', &
     'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', 'ICE-Numerics', &
     'ICE-Numerics', 'ICE-Physics', 'ICE-Physics', 'ICE-IO', 'ICE-IO', 'ICE-IO', 'ICE-Device', 'ICE-Family', &
     'ICE-Device', 'ICE-Parameters', 'ICE-Multigrid', 'ICE-Multigrid', 'ICE-Multigrid', 'ICE-Multigrid', &
-    'ICE-Multigrid']
+    'ICE-Multigrid', &
+    'ICE-Physics', 'ICE-Physics', 'ICE-Physics', 'ICE-Physics', 'ICE-Physics', 'ICE-Physics', 'ICE-Physics', &
+    'ICE-Physics', 'ICE-Physics', 'ICE-Physics', 'ICE-Physics', 'ICE-Physics', 'ICE-Physics', 'ICE-Physics', &
+    'ICE-Physics', 'ICE-IO']
   character(len=24), parameter :: key_name(NKEY) = [character(len=24) :: &
     'iter-threshold', 'time-scheme', 'cfl', 'dt-max', 'tau-factor', 'cfl-rise-threshold', 'time-accurate', &
     'irs', 'irs-beta', 'space-reconstruction', 'flux-limiter', 'shock-detector', 'riemann-solver', &
     'drag', 'heat-transfer', 'ic-format', 'sol-format', 'shell-diter', 'mode', 'closure', 'autotune', &
-    'res-threshold', 'levels', 'level1-iter', 'level2-iter', 'level3-iter', 'level4-iter']
+    'res-threshold', 'levels', 'level1-iter', 'level2-iter', 'level3-iter', 'level4-iter', &
+    'evaporation', 'evaporation-interface', 'evaporation-blowing', 'density', 'specific-heat', 'latent-heat', &
+    'emissivity', 'vapour-molar-mass', 'boiling-temperature', 'Tboil', 'boiling-pressure', 'vapour-specific-heat', &
+    'lewis-number', 'vapour-mass-fraction', 'evaporation-coefficient', 'gas-path']
+  !> (Register_Physics.f90:26-126, Register_IO.f90:46-48)
   character(len=24), parameter :: key_default(NKEY) = [character(len=24) :: &
     '1000000000', 'RK2', '0.5', '1e-4', '1.0', '0', '.true.', '.false.', '0.5', 'first-order', 'none', 'none', &
-    '', 'NoDrag', 'NoHeat', 'tecplot ascii', 'tecplot ascii', '10', 'plain', 'IG', '.false.', &
-    '0.0', '1', '1000000000', '1000000000', '1000000000', '1000000000']
+    '', 'none', 'none', 'tecplot ascii', 'tecplot ascii', '10', 'plain', 'IG', '.false.', &
+    '0.0', '1', '1000000000', '1000000000', '1000000000', '1000000000', &
+    'none', 'VLE', 'none', '2700.0', '1598.0', '1.08e7', '0.0', '26.98', '2792.0', '2792.0', '101325.0', '0.0', &
+    '1.0', '0.0', '1.0', 'INPUT/']
 
   type, public :: registry_t
     character(len=64) :: value(NKEY)
+    logical :: set(NKEY) = .false.      !< given in input.ini
     integer :: nfam = 0
     character(len=8) :: closure(14) = 'IG'
   end type
 
-  public :: read_registry, reg_str, reg_real, reg_int, reg_bool, config_from_registry
+  public :: read_registry, reg_str, reg_real, reg_int, reg_bool, reg_set, config_from_registry
   public :: read_ic, read_bc, read_material, block_metrics, write_solution, refuse, coarse_blocks
 
 contains
@@ -105,6 +115,7 @@ contains
         do n = 1, NKEY
           if (trim(key_sec(n)) == trim(sec) .and. trim(key_name(n)) == trim(key)) then
             reg%value(n) = trim(val)
+            reg%set(n) = .true.
             found = .true.
           end if
         end do
@@ -145,6 +156,16 @@ contains
     if (ios /= 0) call refuse('Read_Inifile', key//' is not an integer')
   end function
 
+  logical function reg_set(reg, key)
+    type(registry_t), intent(in) :: reg
+    character(len=*), intent(in) :: key
+    integer :: n
+    reg_set = .false.
+    do n = 1, NKEY
+      if (trim(key_name(n)) == key) reg_set = reg_set .or. reg%set(n)
+    end do
+  end function
+
   logical function reg_bool(reg, key)
     type(registry_t), intent(in) :: reg
     character(len=*), intent(in) :: key
@@ -153,20 +174,107 @@ contains
     reg_bool = any(trim(v) == [character(len=8) :: '.true.', 'true', 'T', '1', 'yes', '.TRUE.'])
   end function
 
-  !> ice_config_t and the families from the registry (Assign_Setup.f90:52-117);
-  !> an uncoupled run (the gas fields of a coupled one come from the coupler).
-  subroutine config_from_registry(reg, ndir, rho_mat, cs, cfg, fams)
+  !> ice_config_t and the families from the registry (Assign_Setup.f90:52-117):
+  !> coupled when the gas field is present (owcoupled, Assign_Setup.f90:50-54),
+  !> with the exchange models of [ICE-Physics] and the material and vapour keys
+  !> packed as Setup_Materials does (Setup_Materials.f90:140-170).
+  subroutine config_from_registry(reg, ndir, rho_mat, cs, coupled, cfg, fams)
     type(registry_t), intent(in) :: reg
     integer, intent(in) :: ndir
     real(R8), intent(in) :: rho_mat, cs
+    logical, intent(in) :: coupled
     type(ice_config_t), intent(out) :: cfg
     type(ice_family_t), allocatable, intent(out) :: fams(:)
     character(len=64) :: w
-    integer :: p
+    integer :: p, evap, intf, blow
+    real(R8) :: ep(8), tboil, pboil
+    real(R8), parameter :: RU = 8314.46_R8, PATM = 101325._R8   ! Lib_Evaporation.f90:22, Parameters_m
     character(len=10), parameter :: lims(0:10) = [character(len=10) :: 'none', 'minmod', 'vanalbada', &
       'vanleer', 'ospre', 'umist', 'osher', 'sweby', 'mc', 'koren', 'superbee']
-    if (trim(reg_str(reg, 'drag')) /= 'NoDrag' .or. trim(reg_str(reg, 'heat-transfer')) /= 'NoHeat') &
-      call refuse('Assign_Setup', 'a coupled run needs the gas fields of a coupler (ice_gpu_put_gas)')
+    character(len=16), parameter :: drags(13) = [character(len=16) :: 'Newton', 'Stokes', 'Schlichting', &
+      'Schiller-Naumann', 'Wen-Yu', 'Putnam', 'Clift-Gauvin', 'Morsi-Alexander', 'Carlson-Hoglund', 'Henderson', &
+      'Crowe', 'Hermsen', 'NoDrag']
+    character(len=16), parameter :: heats(8) = [character(len=16) :: 'Stokes', 'JAXA1', 'JAXA2', 'JAXA3', &
+      'JAXA4', 'Ranz-Marshall', 'Kavanau-Drake', 'NoHeat']
+    character(len=8), parameter :: evaps(5) = [character(len=8) :: 'd2-law', 'CEM', 'CEM-B', 'ASM', 'TC']
+    cfg%coupled = merge(1, 0, coupled)
+    evap = 0; intf = 0; blow = 0
+    ep = 0._R8
+    if (coupled) then
+      ! assign_drag / assign_heat (Lib_Drag.f90:17-80, Lib_Heat.f90:18-62)
+      w = reg_str(reg, 'drag')
+      if (trim(w) == 'none') call refuse('assign_drag', 'drag is not set: [ICE-Physics] drag is required for a '// &
+                                         'coupled run')
+      cfg%drag_select = 0
+      do p = 1, 13
+        if (trim(w) == trim(drags(p))) cfg%drag_select = p
+      end do
+      if (cfg%drag_select == 0) call refuse('assign_drag', 'Wrong drag input ---> '//trim(w))
+      w = reg_str(reg, 'heat-transfer')
+      if (trim(w) == 'none') call refuse('assign_heat', 'heat-transfer is not set: [ICE-Physics] heat-transfer is '// &
+                                         'required for a coupled run')
+      cfg%heat_select = 0
+      do p = 1, 8
+        if (trim(w) == trim(heats(p))) cfg%heat_select = p
+      end do
+      if (cfg%heat_select == 0) call refuse('assign_heat', 'Wrong heat input ---> '//trim(w))
+      w = reg_str(reg, 'evaporation')
+      if (trim(w) /= 'none') then
+        do p = 1, 5
+          if (trim(w) == trim(evaps(p))) evap = p
+        end do
+        if (evap == 0) call refuse('assign_evaporation', 'unknown evaporation model '//trim(w))
+      end if
+      w = reg_str(reg, 'evaporation-interface')
+      if (trim(w) == 'LK') then
+        intf = 1
+      else if (trim(w) /= 'VLE') then
+        call refuse('assign_interface', 'evaporation-interface is VLE or LK, not '//trim(w))
+      end if
+      w = reg_str(reg, 'evaporation-blowing')
+      if (trim(w) == 'LK') then
+        blow = 1
+      else if (trim(w) /= 'none') then
+        call refuse('assign_blowing', 'evaporation-blowing is LK or none, not '//trim(w))
+      end if
+      if (evap == 0) then
+        intf = 0
+        blow = 0
+      end if
+      if (evap > 0) then
+        ! Setup_Materials.f90:140-170: the keys an evaporating material needs, then ep
+        if (.not. reg_set(reg, 'vapour-specific-heat')) call refuse('Setup_Materials', &
+          'evaporation requires vapour-specific-heat (0 for the gas cp): give it in [ICE-Physics]')
+        if (.not. (reg_set(reg, 'boiling-temperature') .or. reg_set(reg, 'Tboil'))) call refuse('Setup_Materials', &
+          'evaporation requires boiling-temperature (alias Tboil) in [ICE-Physics]')
+        if (reg_set(reg, 'boiling-temperature') .and. reg_set(reg, 'Tboil')) call refuse('Setup_Materials', &
+          'boiling-temperature and its alias Tboil are both given: give one of them')
+        if (.not. reg_set(reg, 'lewis-number')) call refuse('Setup_Materials', &
+          'evaporation requires lewis-number in [ICE-Physics]')
+        if (.not. reg_set(reg, 'vapour-mass-fraction')) call refuse('Setup_Materials', &
+          'evaporation requires vapour-mass-fraction in [ICE-Physics]')
+        if (.not. reg_real(reg, 'vapour-molar-mass') > 0._R8) call refuse('Setup_Materials', &
+          'vapour-molar-mass = 0 needs an Mv column in the property table')
+        tboil = reg_real(reg, 'boiling-temperature')
+        if (reg_set(reg, 'Tboil')) tboil = reg_real(reg, 'Tboil')
+        pboil = reg_real(reg, 'boiling-pressure')
+        ep(1) = reg_real(reg, 'vapour-molar-mass')
+        ep(2) = reg_real(reg, 'latent-heat')
+        ep(3) = reg_real(reg, 'vapour-specific-heat')
+        ep(4) = reg_real(reg, 'lewis-number')
+        ep(5) = reg_real(reg, 'vapour-mass-fraction')
+        ep(6) = ep(2) * ep(1) / RU
+        ep(7) = 1._R8 / tboil
+        ep(8) = reg_real(reg, 'evaporation-coefficient')
+        ! boiling-temperature given at boiling-pressure: the line anchored at one atmosphere (invTboil_at)
+        if (pboil /= PATM) then
+          ep(7) = 1._R8 / tboil + log(pboil / PATM) / ep(6)
+          if (.not. ep(7) > 0._R8) call refuse('Setup_Materials', 'boiling-pressure and boiling-temperature give '// &
+            'no one-atmosphere boiling point on the Clausius-Clapeyron line')
+        end if
+      end if
+    end if
+    cfg%blow_select = blow
     w = reg_str(reg, 'time-scheme')
     select case (trim(w))
     case ('euler'); cfg%nrk = 1
@@ -195,7 +303,6 @@ contains
       if (cfg%limiter < 0) call refuse('Assign_Setup', 'unknown flux-limiter '//trim(w))
     end if
     cfg%shock_detector = merge(1, 0, trim(reg_str(reg, 'shock-detector')) == 'Jameson')
-    cfg%coupled = 0
     cfg%mode = merge(ICE_MODE_BATCHED, ICE_MODE_PLAIN, trim(reg_str(reg, 'mode')) == 'batched')
     allocate(fams(reg%nfam))
     do p = 1, reg%nfam
@@ -211,6 +318,11 @@ contains
       if (trim(w) == 'HLLE') fams(p)%riemann = ICE_HLLE
       fams(p)%rho_mat = rho_mat
       fams(p)%cs = cs
+      fams(p)%lv = reg_real(reg, 'latent-heat')
+      fams(p)%emiss = min(reg_real(reg, 'emissivity'), 1._R8)
+      fams(p)%evap_select = evap
+      fams(p)%intf_select = intf
+      fams(p)%ep = ep
     end do
   end subroutine
 

@@ -16,7 +16,9 @@ program ice_driver
     type(host_block_t), allocatable :: b(:)
     type(c_ptr) :: ctx = c_null_ptr
   end type
-  character(len=1024) :: dir, arg, bcpath
+  character(len=1024) :: dir, arg, bcpath, gaspath
+  type(host_block_t), allocatable :: gas(:)
+  logical :: coupled
   type(registry_t) :: reg
   type(level_t) :: lev(MAXLEV)
   type(host_block_t), allocatable :: blocks(:)
@@ -52,12 +54,24 @@ program ice_driver
   if (mgl < 1 .or. mgl > MAXLEV) call refuse('Register_Multigrid_Levels', 'levels must be 1 to 4 here')
   call read_ic(trim(dir)//'/INPUT/part-ic.tec', nc, blocks)
   call read_material(trim(dir)//'/INPUT/part-properties.dat', rho_mat, cs, t_min, prop_cols, prop_flags)
+  ! one-way coupling when the gas field is there (Assign_Setup.f90:50-54): gas.tec of
+  ! gas-path, ten cell variables rho u v w T R gam k mu dt (setup_gas, Mod_Phase.f90:46-58)
+  gaspath = trim(dir)//'/'//trim(reg_str(reg, 'gas-path'))//'gas.tec'
+  inquire(file=trim(gaspath), exist=coupled)
+  if (coupled) then
+    call read_ic(trim(gaspath), 10, gas)
+    if (size(gas) /= size(blocks)) call refuse('setup_gas', 'the gas field has another number of blocks')
+    do b = 1, size(blocks)
+      if (any(gas(b)%dim /= blocks(b)%dim)) call refuse('setup_gas', 'the gas field of block '//itoa_m(b)// &
+                                                        ' has other dimensions')
+    end do
+  end if
   nb = size(blocks)
   ndir = 0
   do n = 1, 3
     if (any([(blocks(b)%dim(n) > 1, b = 1, nb)])) ndir = ndir + 1
   end do
-  call config_from_registry(reg, ndir, rho_mat, cs, cfg, fams)
+  call config_from_registry(reg, ndir, rho_mat, cs, coupled, cfg, fams)
   do b = 1, nb
     call block_metrics(blocks(b))
   end do
@@ -87,6 +101,11 @@ program ice_driver
     ndonor = 0
     if (any(reci(R_TYPE, :) == 102)) ndonor = size(weight)
     call check(ice_gpu_set_boundaries(ctx, size(reci, 2), reci, recd, ndonor, donor, weight), 'set_boundaries')
+    if (coupled) then
+      do b = 1, nb
+        call put_gas(b, lev(m)%b(b)%dim)
+      end do
+    end if
     if (m == 1) then
       do b = 1, nb
         call check(ice_gpu_put_state(ctx, b, 1, lev(1)%b(b)%prim), 'put_state')
@@ -180,6 +199,28 @@ program ice_driver
   end do
 
 contains
+
+  !> The gas of block b (set_gas, Mod_Exchange.f90:63-81): prim (5, ni, nj, nk) and
+  !> R, gam, k, mu from the gas field's cells; a coarse level takes none (it runs
+  !> uncoupled levels only)
+  subroutine put_gas(b, d)
+    integer, intent(in) :: b, d(3)
+    real(R8), allocatable :: g5(:,:,:,:), gR(:,:,:), gG(:,:,:), gK(:,:,:), gM(:,:,:)
+    if (any(d /= gas(b)%dim)) call refuse('setup_gas', 'a coupled run on several grid levels is not supported')
+    allocate(g5(5, d(1), d(2), d(3)))
+    g5 = gas(b)%prim(1:5, 1:d(1), 1:d(2), 1:d(3))
+    gR = gas(b)%prim(6, 1:d(1), 1:d(2), 1:d(3))
+    gG = gas(b)%prim(7, 1:d(1), 1:d(2), 1:d(3))
+    gK = gas(b)%prim(8, 1:d(1), 1:d(2), 1:d(3))
+    gM = gas(b)%prim(9, 1:d(1), 1:d(2), 1:d(3))
+    call check(ice_gpu_put_gas(ctx, b, g5, gR, gG, gK, gM), 'put_gas')
+  end subroutine
+
+  function itoa_m(i) result(s)
+    integer, intent(in) :: i
+    character(len=12) :: s
+    write(s, '(I0)') i
+  end function
 
   !> The 400-inflow lines of the steps just run, in ICE's text (Mod_Inflow400.f90:118-123)
   subroutine print_inflow400_lines(level)

@@ -163,24 +163,22 @@ def uniform_metrics(dims, dx: float) -> dict:
     """The metrics() of a uniform Cartesian block of cubes of side dx, in closed form
     (equal to metrics() on such a grid to rounding), for blocks too large for the
     node-based construction: vol = dx^3, lengths dx, M = I/dx, unit normals along
-    the axes with area dx^2."""
+    the axes with area dx^2.  Each array is flat in its Fortran layout, built by
+    repeating the one cell's (or face's) entries."""
     ni, nj, nk = dims
-    f = lambda a: np.ascontiguousarray(a.ravel(order="F"))  # noqa: E731
-    vol = np.full((ni, nj, nk), dx * dx * dx)
-    dl = np.full((3, ni + 2, nj + 2, nk + 2), dx)
-    M = np.zeros((3, 3, ni, nj, nk))
-    for d in range(3):
-        M[d, d] = 1.0 / dx
+    vol = np.full(ni * nj * nk, dx * dx * dx)
+    dl = np.full(3 * (ni + 2) * (nj + 2) * (nk + 2), dx)
+    M = np.tile(np.eye(3).ravel(order="F") / dx, ni * nj * nk)
 
-    def face(d, shape):
-        a = np.zeros((4,) + shape)
-        a[d] = 1.0
-        a[3] = dx * dx
-        return a
+    def face(d, n):
+        e = np.zeros(4)
+        e[d] = 1.0
+        e[3] = dx * dx
+        return np.tile(e, n)
 
-    return {"dims": (ni, nj, nk), "vol": f(vol), "dl": f(dl), "M": f(M),
-            "face_i": f(face(0, (ni + 1, nj, nk))), "face_j": f(face(1, (ni, nj + 1, nk))),
-            "face_k": f(face(2, (ni, nj, nk + 1)))}
+    return {"dims": (ni, nj, nk), "vol": vol, "dl": dl, "M": M,
+            "face_i": face(0, (ni + 1) * nj * nk), "face_j": face(1, ni * (nj + 1) * nk),
+            "face_k": face(2, ni * nj * (nk + 1))}
 
 
 def smooth_field_axes(rng, x, y, z, modes: int = 16, kmax: int = 4):
@@ -196,3 +194,22 @@ def smooth_field_axes(rng, x, y, z, modes: int = 16, kmax: int = 4):
         ez = np.exp(1j * 2 * np.pi * k[2] * np.asarray(z))
         f += amp * (ex[:, None, None] * (ey[:, None] * ez[None, :])[None]).imag
     return f
+
+
+def smooth_field_product(rng, x, y, z, modes: int = 16, kmax: int = 4):
+    """smooth_field_axes with the same draws, summed over the modes as one matrix
+    product: sin(a + b) = sin a cos b + cos a sin b splits each mode into an
+    x-factor times a (y, z)-factor, so the field is a (ny nz, 2 modes) by
+    (2 modes, nx) product.  The values equal smooth_field_axes' to rounding; the
+    (nx, ny, nz) result is laid out in Fortran order (x fastest)."""
+    x, y, z = (np.asarray(t, float) for t in (x, y, z))
+    A = np.empty((2 * modes, len(x)))
+    B = np.empty((len(z) * len(y), 2 * modes))
+    for m in range(modes):
+        k = rng.integers(1, kmax + 1, size=3) * rng.choice([-1, 1], size=3)
+        amp, ph = rng.normal(), rng.uniform(0, 2 * np.pi)
+        ax = 2 * np.pi * k[0] * x + ph
+        ezy = np.exp(1j * 2 * np.pi * k[2] * z)[:, None] * np.exp(1j * 2 * np.pi * k[1] * y)[None, :]
+        A[2 * m], A[2 * m + 1] = amp * np.sin(ax), amp * np.cos(ax)
+        B[:, 2 * m], B[:, 2 * m + 1] = ezy.real.ravel(), ezy.imag.ravel()
+    return (B @ A).reshape(len(z), len(y), len(x)).T

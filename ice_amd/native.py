@@ -30,11 +30,12 @@ RD_MASSFLUX, RD_VELOCITY, RD_TEMPERATURE, RD_RADIUS, RD_ALPHA, RD_BETA, RD_GFIXE
 REC_ND = 7
 
 NCOND = {MK: 6, IG: 7, AG: 12}
+NFROZEN_MAX = 6
 
 
 class Family(C.Structure):
     _fields_ = [("closure", C.c_int32), ("riemann", C.c_int32), ("nfrozen", C.c_int32),
-                ("frozen", C.c_int32 * 4), ("evap_select", C.c_int32), ("intf_select", C.c_int32),
+                ("frozen", C.c_int32 * NFROZEN_MAX), ("evap_select", C.c_int32), ("intf_select", C.c_int32),
                 ("surrogate", C.c_int32), ("rho_mat", C.c_double), ("cs", C.c_double), ("lv", C.c_double),
                 ("emiss", C.c_double), ("ep", C.c_double * NEP)]
 
@@ -84,6 +85,7 @@ SIGNATURES = {
     "ice_gpu_put_gas": [C.c_void_p, C.c_int, _DP, _DP, _DP, _DP, _DP],
     "ice_gpu_put_gas_yinf": [C.c_void_p, C.c_int, _DP],
     "ice_gpu_put_tau": [C.c_void_p, C.c_int, C.c_int, _DP],
+    "ice_gpu_get_tau": [C.c_void_p, C.c_int, C.c_int, _DP],
     "ice_gpu_step": [C.c_void_p, C.c_int, C.POINTER(Status)],
     "ice_gpu_set_keep_source": [C.c_void_p, C.c_int],
     "ice_gpu_get_source": [C.c_void_p, C.c_int, C.c_int, _DP, _DP],
@@ -109,6 +111,13 @@ SIGNATURES = {
     "ice_gpu_inflow400_lines": [C.c_void_p, C.POINTER(Inflow400Line), C.c_int, C.POINTER(C.c_int32),
                                 C.POINTER(C.c_int64)],
     "ice_gpu_inflow400_totals": [C.c_void_p, C.POINTER(Inflow400Total), C.c_int, C.POINTER(C.c_int32)],
+    "ice_gpu_fill_ghosts": [C.c_void_p],
+    "ice_gpu_put_imports": [C.c_void_p, C.c_int, C.c_int, C.c_void_p],
+    "ice_gpu_add_births": [C.c_void_p, C.c_int, C.c_int, _IP, _DP],
+    "ice_gpu_set_mode": [C.c_void_p, C.c_int],
+    "ice_gpu_set_surrogate": [C.c_void_p, C.c_int, C.c_int],
+    "ice_gpu_set_counters": [C.c_void_p, C.c_int32, C.c_double, C.c_double],
+    "ice_gpu_get_counters": [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)],
     "ice_gpu_last_error": [C.c_void_p, C.c_char_p, C.c_int],
     "ice_gpu_destroy": [C.c_void_p],
 }

@@ -33,6 +33,7 @@ extern "C" {
 #define ICE_GC        2    /* ghost layers, ICE_Global_m gc (Global_m.f90:9) */
 #define ICE_NCOND_MAX 14   /* Global_m.f90:24 */
 #define ICE_NEP       8    /* packed vapour properties, Lib_Evaporation.f90:15-19 */
+#define ICE_NFROZEN_MAX 6  /* frozen slots: v, w and AG's P12, P13, P23 (Mod_Freeze.f90:117-128) */
 
 /* Closures (Lib_Model.f90:105-316).  nc = 6 / 7 / 12. */
 enum { ICE_MK = 0, ICE_IG = 1, ICE_AG = 2 };
@@ -50,7 +51,7 @@ typedef struct {
   int32_t closure;          /* ICE_MK / ICE_IG / ICE_AG                                  */
   int32_t riemann;          /* ICE_SAUREL / ICE_RUSANOV / ICE_HLLE                       */
   int32_t nfrozen;          /* slots not integrated (Mod_Freeze.f90:37-141)              */
-  int32_t frozen[4];        /* their 1-based slot numbers                                */
+  int32_t frozen[ICE_NFROZEN_MAX]; /* their 1-based slot numbers                         */
   int32_t evap_select;      /* 0 none, 1 d2-law, 2 CEM, 3 CEM-B, 4 ASM, 5 TC             */
   int32_t intf_select;      /* 0 VLE, 1 LK                                               */
   int32_t surrogate;        /* 1: seed the LK interface solve with the learned model     */
@@ -137,8 +138,11 @@ int  ice_gpu_put_gas(ice_gpu_ctx* ctx, int b, const double* gprim, const double*
 /* Far-field vapour mass fraction per cell, replacing ep(iYinf) (cfg.var_yinf = 1). */
 int  ice_gpu_put_gas_yinf(ice_gpu_ctx* ctx, int b, const double* yinf);
 
-/* Relaxation time of family p (ni,nj,nk) before the first step; the stages write it. */
+/* Relaxation time of family p (ni,nj,nk) before the first step; the stages write it,
+ * and the next step's dt pass reads it (Mod_dt.f90:94-98): a restart that puts it
+ * back goes on as the uninterrupted run. */
 int  ice_gpu_put_tau(ice_gpu_ctx* ctx, int b, int p, const double* tau);
+int  ice_gpu_get_tau(ice_gpu_ctx* ctx, int b, int p, double* tau);
 
 /* nsteps calls of Explicit_Step, all on the device; status after the last one.
  * Returns non-zero when a step hit a fatal state (st->bad_state / bad_recon). */
@@ -280,6 +284,38 @@ int  ice_gpu_inflow400_lines(ice_gpu_ctx* ctx, ice_inflow400_line_t* out, int ma
 /* The totals of every boundary that let particles in, in the reference's report
  * order (family, block, face): *n of them, at most max. */
 int  ice_gpu_inflow400_totals(ice_gpu_ctx* ctx, ice_inflow400_total_t* out, int max, int32_t* n);
+
+/* compute_ghost and fill_second_ghost (Lib_Ghost.f90:29-72, 270-308): both ghost
+ * layers of every family on its current state, as the next stage fills them
+ * first; ice_gpu_get_state then returns them. */
+int  ice_gpu_fill_ghosts(ice_gpu_ctx* ctx);
+
+/* ---- coupler births (Mod_Exchange.f90:225-324).  The rates in force on family p
+ * in block b, in the conservative slots (nc, ni, nj, nk) per unit volume and time,
+ * as clear_import_rates + add_birth_rate / add_absorption_rate / add_removal_rate
+ * leave them (the packets are the coupler's: birth_packet, absorption_packet);
+ * every stage adds them to the residual over the cell's dt (add_births,
+ * Lib_Residual.f90:164-210).  NULL clears the block's. */
+int  ice_gpu_put_imports(ice_gpu_ctx* ctx, int b, int p, const double* imports);
+/* add_birth (Mod_Exchange.f90:283-300): n packets of family p at once, packet m
+ * born in cell cells[4m..4m+3] = (b, i, j, k) with pk[6m..6m+5] = [mass, momentum
+ * (3), energy, number] per unit volume, added to the cell's conservative state;
+ * the packets of one cell in their order. */
+int  ice_gpu_add_births(ice_gpu_ctx* ctx, int p, int n, const int32_t* cells, const double* pk);
+
+/* ---- run control (no counterpart in the reference, which fixes these at set-up:
+ * Assign_Setup.f90:60-117).  Execution mode of the following steps, as cfg.mode
+ * at create; the interface solve of family p, as ice_family_t.surrogate. */
+int  ice_gpu_set_mode(ice_gpu_ctx* ctx, int mode);
+int  ice_gpu_set_surrogate(ice_gpu_ctx* ctx, int p, int mode);
+
+/* ---- restart (Wrap_Setup.f90:158-214): the counters a solution stores for its
+ * restart (IO_Solution.f90:20-30) -- grid%iter (ICE_ITERATION, which drives the
+ * CFL ramp of Mod_dt.f90:83-85), grid%time, and the residual reference of
+ * res-threshold (ICE_RES_REFERENCE).  Set them after put_state (and put_tau) to
+ * resume a run as the uninterrupted run goes on. */
+int  ice_gpu_set_counters(ice_gpu_ctx* ctx, int32_t iter, double time, double res_reference);
+int  ice_gpu_get_counters(ice_gpu_ctx* ctx, int32_t* iter, double* time, double* res_reference);
 
 int  ice_gpu_last_error(ice_gpu_ctx* ctx, char* buf, int len);
 void ice_gpu_destroy(ice_gpu_ctx* ctx);

@@ -27,7 +27,7 @@ namespace {
 static const int GUIDE[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};  // Parameters_m.f90:16-21
 
 struct Family {
-  int model, solver, nc, nbase, nfrozen, frozen[4];
+  int model, solver, nc, nbase, nfrozen, frozen[ICE_NFROZEN_MAX];
   Mat mat;
   std::vector<double> prop;   // the material's property table (mat.tab points here)
   int surrogate = 0;          // 1: network seed, 2: equilibrium seed
@@ -44,6 +44,7 @@ struct Rec {
 struct Phase {  // ICE_cond_phase_type (Advanced_Types_m.f90:46-84), padded layout
   std::vector<double> prim, prim_old, residual, source, beta, dt, tau;
   std::vector<double> source_step;  // add_stage_source's running sum (Mod_Explicit.f90:257-281)
+  std::vector<double> imports;      // rates in force a coupler set (add_births), nc per cell, or empty
   std::vector<char> inj;
 };
 
@@ -903,6 +904,9 @@ void one_step(orc_ctx* c) {
               const double* s = &ph.source[id * nc];
               double vol = B.vol[id], dt = ph.dt[id];
               for (int v = 0; v < nc; ++v) r[v] = (r[v] / vol + (cf.coupled ? s[v] : 0.0)) * dt;
+              // add_births (Lib_Residual.f90:164-210)
+              if (!ph.imports.empty())
+                for (int v = 0; v < nc; ++v) r[v] = r[v] + ph.imports[id * nc + v] * dt;
               for (int f = 0; f < F.nfrozen; ++f) r[F.frozen[f] - 1] = 0.0;
             }
       }
@@ -1001,7 +1005,7 @@ int orc_create(int /*device*/, const ice_config_t* cfg, const ice_family_t* fams
     F.nc = ncond_of(F.model);
     F.nbase = F.nc;
     F.nfrozen = fams[p].nfrozen;
-    for (int m = 0; m < 4; ++m) F.frozen[m] = fams[p].frozen[m];
+    for (int m = 0; m < ICE_NFROZEN_MAX; ++m) F.frozen[m] = fams[p].frozen[m];
     F.mat.rho = fams[p].rho_mat; F.mat.cs = fams[p].cs; F.mat.lv = fams[p].lv; F.mat.emiss = fams[p].emiss;
     for (int m = 0; m < 8; ++m) F.mat.ep[m] = fams[p].ep[m];
     F.mat.evap = fams[p].evap_select; F.mat.intf = fams[p].intf_select;
@@ -1153,6 +1157,19 @@ int orc_put_tau(orc_ctx* c, int b, int p, const double* tau) {
   return 0;
 }
 
+int orc_get_tau(orc_ctx* c, int b, int p, double* tau) {
+  if (b < 1 || b > (int)c->blk.size() || p < 1 || p > c->cfg.ngroups) {
+    c->err = "[ICE::get_tau] no block / family";
+    return 1;
+  }
+  Blk& B = c->blk[b - 1];
+  for (int kk = 1; kk <= B.nk; ++kk)
+    for (int j = 1; j <= B.nj; ++j)
+      for (int i = 1; i <= B.ni; ++i)
+        tau[(size_t)(i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (kk - 1))] = B.ph[p - 1].tau[B.id(i, j, kk)];
+  return 0;
+}
+
 int orc_step(orc_ctx* c, int nsteps, ice_status_t* st) {
   c->res_stopped = 0;
   for (int s = 0; s < nsteps; ++s) {
@@ -1174,6 +1191,105 @@ int orc_step(orc_ctx* c, int nsteps, ice_status_t* st) {
     st->dtglobal = c->dtglobal; st->time = c->time;
   }
   return (c->bad_state || c->bad_recon) ? 2 : 0;
+}
+
+// compute_ghost and fill_second_ghost (Lib_Ghost.f90:29-72, 270-308) of every family
+int orc_fill_ghosts(orc_ctx* c) {
+  for (int p = 1; p <= c->cfg.ngroups; ++p) {
+    const std::vector<int>& g = c->grp[p - 1];
+    for (int ii = 0; ii < (int)g.size(); ++ii) ghost_record(c, g[ii]);
+    for (int ii = 0; ii < (int)g.size(); ++ii) second_ghost_record(c, g[ii]);
+  }
+  return 0;
+}
+
+// the rates in force on family p in block b (Mod_Exchange.f90:242-324), (nc, ni, nj, nk)
+int orc_put_imports(orc_ctx* c, int b, int p, const double* imports) {
+  if (b < 1 || b > (int)c->blk.size() || p < 1 || p > c->cfg.ngroups) {
+    c->err = "[ICE::put_imports] no block / family";
+    return 1;
+  }
+  Blk& B = c->blk[b - 1];
+  Phase& ph = B.ph[p - 1];
+  int nc = c->fam[p - 1].nc;
+  for (auto& BB : c->blk)
+    if (BB.ph[p - 1].imports.empty()) BB.ph[p - 1].imports.assign(BB.npad * nc, 0.0);
+  for (int k = 1; k <= B.nk; ++k)
+    for (int j = 1; j <= B.nj; ++j)
+      for (int i = 1; i <= B.ni; ++i) {
+        size_t s = (size_t)(i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (k - 1));
+        for (int v = 0; v < nc; ++v) ph.imports[B.id(i, j, k) * nc + v] = imports ? imports[s * nc + v] : 0.0;
+      }
+  return 0;
+}
+
+// add_birth (Mod_Exchange.f90:283-300) of n packets, in their order
+int orc_add_births(orc_ctx* c, int p, int n, const int32_t* cells, const double* pk) {
+  if (p < 1 || p > c->cfg.ngroups) {
+    c->err = "[ICE::add_births] no family";
+    return 1;
+  }
+  const Family& F = c->fam[p - 1];
+  int nc = F.nc;
+  for (int m = 0; m < n; ++m) {
+    const int* x = cells + 4 * (size_t)m;
+    if (x[0] < 1 || x[0] > (int)c->blk.size()) {
+      c->err = "[ICE::add_births] no block";
+      return 1;
+    }
+    Blk& B = c->blk[x[0] - 1];
+    if (x[1] < 1 || x[1] > B.ni || x[2] < 1 || x[2] > B.nj || x[3] < 1 || x[3] > B.nk) {
+      c->err = "[ICE::add_births] cell out of its block";
+      return 1;
+    }
+    double* q = &B.ph[p - 1].prim[B.id(x[1], x[2], x[3]) * nc];
+    double u[ICE_NCOND_MAX];
+    p2c(F.model, q, u, F.mat);
+    const double* k = pk + 6 * (size_t)m;
+    for (int v = 0; v < 4; ++v) u[v] = u[v] + k[v];
+    u[nc - 2] = u[nc - 2] + k[4];
+    u[nc - 1] = u[nc - 1] + k[5];
+    c2p(F.model, u, q, F.mat);
+  }
+  return 0;
+}
+
+// the oracle has one path: the execution mode changes nothing
+int orc_set_mode(orc_ctx* c, int mode) {
+  if (mode != ICE_MODE_PLAIN && mode != ICE_MODE_BATCHED) {
+    c->err = "[ICE::set_mode] mode is ICE_MODE_PLAIN (0) or ICE_MODE_BATCHED (1)";
+    return 1;
+  }
+  c->cfg.mode = mode;
+  return 0;
+}
+
+int orc_set_surrogate(orc_ctx* c, int p, int mode) {
+  if (p < 1 || p > c->cfg.ngroups || mode < 0 || mode > 2) {
+    c->err = "[ICE::set_surrogate] family or mode out of range";
+    return 1;
+  }
+  c->fam[p - 1].surrogate = mode;
+  return 0;
+}
+
+// restart counters (Wrap_Setup.f90:158-214, IO_Solution.f90:20-30)
+int orc_set_counters(orc_ctx* c, int32_t iter, double time, double res_reference) {
+  if (iter < 0 || !std::isfinite(time) || !(res_reference >= 0.0)) {
+    c->err = "[ICE::set_counters] iter >= 0, a finite time and res_reference >= 0";
+    return 1;
+  }
+  c->iter = iter;
+  c->time = time;
+  c->res_reference = res_reference;
+  return 0;
+}
+
+int orc_get_counters(orc_ctx* c, int32_t* iter, double* time, double* res_reference) {
+  if (iter) *iter = c->iter;
+  if (time) *time = c->time;
+  if (res_reference) *res_reference = c->res_reference;
+  return 0;
 }
 
 int orc_get_source(orc_ctx* c, int b, int p, double* source, double* tau) {

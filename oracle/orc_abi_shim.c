@@ -28,6 +28,14 @@ int orc_prolong(orc_ctx*, orc_ctx*);
 int orc_set_res_threshold(orc_ctx*, double);
 int orc_res_stop(orc_ctx*, int32_t*, double*);
 int orc_inflow400_totals(orc_ctx*, ice_inflow400_total_t*, int, int32_t*);
+int orc_fill_ghosts(orc_ctx*);
+int orc_get_tau(orc_ctx*, int, int, double*);
+int orc_put_imports(orc_ctx*, int, int, const double*);
+int orc_add_births(orc_ctx*, int, int, const int32_t*, const double*);
+int orc_set_mode(orc_ctx*, int);
+int orc_set_surrogate(orc_ctx*, int, int);
+int orc_set_counters(orc_ctx*, int32_t, double, double);
+int orc_get_counters(orc_ctx*, int32_t*, double*, double*);
 int orc_last_error(orc_ctx*, char*, int);
 void orc_destroy(orc_ctx*);
 
@@ -72,5 +80,15 @@ int ice_gpu_inflow400_lines(ice_gpu_ctx* x, ice_inflow400_line_t* o, int m, int3
 int ice_gpu_inflow400_totals(ice_gpu_ctx* x, ice_inflow400_total_t* o, int m, int32_t* n) {
   return orc_inflow400_totals(C(x), o, m, n);
 }
+int ice_gpu_fill_ghosts(ice_gpu_ctx* x) { return orc_fill_ghosts(C(x)); }
+int ice_gpu_get_tau(ice_gpu_ctx* x, int b, int p, double* t) { return orc_get_tau(C(x), b, p, t); }
+int ice_gpu_put_imports(ice_gpu_ctx* x, int b, int p, const double* im) { return orc_put_imports(C(x), b, p, im); }
+int ice_gpu_add_births(ice_gpu_ctx* x, int p, int n, const int32_t* c, const double* pk) {
+  return orc_add_births(C(x), p, n, c, pk);
+}
+int ice_gpu_set_mode(ice_gpu_ctx* x, int m) { return orc_set_mode(C(x), m); }
+int ice_gpu_set_surrogate(ice_gpu_ctx* x, int p, int m) { return orc_set_surrogate(C(x), p, m); }
+int ice_gpu_set_counters(ice_gpu_ctx* x, int32_t it, double t, double r) { return orc_set_counters(C(x), it, t, r); }
+int ice_gpu_get_counters(ice_gpu_ctx* x, int32_t* it, double* t, double* r) { return orc_get_counters(C(x), it, t, r); }
 int ice_gpu_last_error(ice_gpu_ctx* x, char* buf, int len) { return orc_last_error(C(x), buf, len); }
 void ice_gpu_destroy(ice_gpu_ctx* x) { orc_destroy(C(x)); }

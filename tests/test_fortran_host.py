@@ -103,3 +103,150 @@ def test_driver_on_gpu(tmp_path, name, mode):
     rn, worst = compare(d, ref)
     print("rho/n rel L2", rn, "worst", worst)
     assert rn <= 1e-12 and worst <= (1e-8 if name == "AG" else 1e-12)
+
+
+# ---------------------------------------------------------------- coupled runs
+_DRAGS = ["Newton", "Stokes", "Schlichting", "Schiller-Naumann", "Wen-Yu", "Putnam", "Clift-Gauvin",
+          "Morsi-Alexander", "Carlson-Hoglund", "Henderson", "Crowe", "Hermsen", "NoDrag"]
+_HEATS = ["Stokes", "JAXA1", "JAXA2", "JAXA3", "JAXA4", "Ranz-Marshall", "Kavanau-Drake", "NoHeat"]
+_CLOSURES = {0: "MK", 1: "IG", 2: "AG"}
+_RIEMANN = {0: "Saurel", 1: "Rusanov", 2: "HLLE"}
+_LIMS = ["none", "minmod", "vanalbada", "vanleer", "ospre", "umist", "osher", "sweby", "mc", "koren", "superbee"]
+
+
+def c1_case():
+    """BASELINE C1 at full size: 2,000 cells, IG, Riemann problem, drag only."""
+    from ice_amd import cases
+    case = cases.case_riemann_1d(n=2000, steps=1000)
+    n = 2000
+    dx = 1.0 / n
+    return case, [geometry_nodes(np.linspace(0.0, 1.0, n + 1), [0.0, dx], [0.0, dx])]
+
+
+def c4_reduced():
+    """The evaporation-heavy configuration at 16^3 cells, with one far-field vapour
+    fraction (the reference's Yinf is a material constant), 4 steps."""
+    from ice_amd import cases, geometry
+    case = cases.case_evaporation(n=16, steps=4, box=1.6e-3)
+    case.cfg.var_yinf = 0
+    case.fams[0].ep[4] = 0.1
+    x = np.arange(17) * (1.6e-3 / 16)
+    nodes = geometry_nodes(x, x, x)
+    B = case.blocks[0]
+    B.yinf = None
+    B.geo = geometry.metrics(nodes)
+    return case, [nodes]
+
+
+def geometry_nodes(x, y, z):
+    from ice_amd import geometry
+    return geometry.cartesian_nodes(np.asarray(x, float), np.asarray(y, float), np.asarray(z, float))
+
+
+def write_coupled(d, case, nodes, mode="plain"):
+    """The case in ICE's input files: input.ini with [ICE-Physics], part-ic.tec,
+    part-bc.txt, a constant property table and the gas field INPUT/gas.tec."""
+    from ice_amd import native
+    os.makedirs(os.path.join(d, "INPUT"), exist_ok=True)
+    cfg, fam = case.cfg, case.fams[0]
+    num = {"time-scheme": {1: "euler", 2: "RK2", 3: "RK3"}[cfg.nrk], "cfl": repr(cfg.cfl), "dt-max": repr(cfg.dt_max),
+           "tau-factor": repr(cfg.tau_factor), "time-accurate": ".true." if cfg.time_accurate else ".false.",
+           "space-reconstruction": "MUSCL" if cfg.limiter else "first-order", "flux-limiter": _LIMS[cfg.limiter],
+           "riemann-solver": _RIEMANN[fam.riemann]}
+    phys = {"drag": _DRAGS[cfg.drag_select - 1], "heat-transfer": _HEATS[cfg.heat_select - 1],
+            "latent-heat": repr(fam.lv), "emissivity": repr(fam.emiss)}
+    if fam.evap_select:
+        ep = list(fam.ep)
+        phys.update({"evaporation": ["d2-law", "CEM", "CEM-B", "ASM", "TC"][fam.evap_select - 1],
+                     "evaporation-interface": "LK" if fam.intf_select else "VLE",
+                     "evaporation-blowing": "LK" if cfg.blow_select else "none",
+                     "vapour-molar-mass": repr(ep[0]), "vapour-specific-heat": repr(ep[2]),
+                     "lewis-number": repr(ep[3]), "vapour-mass-fraction": repr(ep[4]),
+                     "boiling-temperature": repr(round(1.0 / ep[6], 9)), "evaporation-coefficient": repr(ep[7])})
+        assert 1.0 / round(1.0 / ep[6], 9) == ep[6]
+    ini = {"ICE-Parameters": {"iter-threshold": str(case.steps)}, "ICE-Numerics": num, "ICE-Physics": phys,
+           "ICE-Family1": {"closure": _CLOSURES[fam.closure]}, "ICE-IO": {"shell-diter": str(case.steps)},
+           "ICE-Device": {"mode": mode}}
+    formats.write_ini(os.path.join(d, "input.ini"), ini)
+    nc = native.NCOND[fam.closure]
+    zones, gzones = [], []
+    for B, nd in zip(case.blocks, nodes):
+        ni, nj, nk = B.dims
+        q = B.prim[0].reshape((nc, ni + 4, nj + 4, nk + 4), order="F")[:, 2:ni + 2, 2:nj + 2, 2:nk + 2]
+        zones.append(formats.Zone("B", nd.shape[1:], nd, np.ascontiguousarray(q), {}))
+        g = B.gas
+        sh = (ni, nj, nk)
+        cells = np.concatenate([g["prim"].reshape((5,) + sh, order="F")] +
+                               [g[k].reshape((1,) + sh, order="F") for k in ("R", "gam", "k", "mu")] +
+                               [np.zeros((1,) + sh)])
+        gzones.append(formats.Zone("B", nd.shape[1:], nd, np.ascontiguousarray(cells), {}))
+    formats.write_tecplot(os.path.join(d, "INPUT", "part-ic.tec"), ["x", "y", "z"] + [f"v{m}" for m in range(nc)],
+                          zones)
+    formats.write_tecplot(os.path.join(d, "INPUT", "gas.tec"),
+                          ["x", "y", "z", "rho", "u", "v", "w", "T", "R", "gamma", "k", "mu", "dt"], gzones)
+    formats.write_bc(os.path.join(d, "INPUT", "part-bc.txt"), case.reci, case.recd, case.donor, case.weight)
+    with open(os.path.join(d, "INPUT", "part-properties.dat"), "w") as fh:
+        fh.write('TITLE = "Mass Thermodynamic Properties"\nVARIABLES = "Temperature", "Cp", "Density", "Enthalpy"\n')
+        fh.write('ZONE T="A"\nI=3, F=POINT\n')
+        for T in (1.0, 2.0, 3.0):
+            fh.write(f"{T} {fam.cs!r} {fam.rho_mat!r} {fam.cs * T!r}\n")
+
+
+def oracle_reference(oracle, case):
+    """The oracle's state after the case's steps, as the driver writes it (nc, cells)"""
+    from ice_amd import cases
+    r = cases.Runner(oracle, case, "orc_")
+    r.step(case.steps)
+    out = [r.state(b, 1) for b in range(1, len(case.blocks) + 1)]
+    r.close()
+    return out
+
+
+def field_of(d):
+    _, out = formats.read_tecplot(os.path.join(d, "OUTPUT", "part-field.tec"))
+    return [z.cells for z in out]
+
+
+@pytest.mark.parametrize("which", ["C1", "C4"])
+def test_driver_coupled_on_oracle_abi(tmp_path, oracle, which):
+    """The BASELINE C1 configuration (2,000 cells, 1,000 RK3 steps, drag) and a
+    reduced C4 (CEM with the Langmuir-Knudsen interface, Stefan blowing, radiation)
+    through the Fortran host over the oracle's C-ABI: the same state as the oracle
+    run from Python, to 1e-12."""
+    case, nodes = c1_case() if which == "C1" else c4_reduced()
+    d = str(tmp_path / "case")
+    write_coupled(d, case, nodes)
+    p = run_driver(d, {"LD_LIBRARY_PATH": ABI, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS", "8")})
+    print(p.stdout[-600:], p.stderr[-600:])
+    assert p.returncode == 0
+    got = field_of(d)
+    ref = oracle_reference(oracle, case)
+    for g, r in zip(got, ref):
+        assert rel_l2(g, r) <= 1e-12
+
+
+def test_driver_coupled_refuses_missing_models(tmp_path):
+    case, nodes = c1_case()
+    d = str(tmp_path / "case")
+    write_coupled(d, case, nodes)
+    ini = formats.read_ini(os.path.join(d, "input.ini"))
+    del ini["ICE-Physics"]["drag"]
+    formats.write_ini(os.path.join(d, "input.ini"), ini)
+    p = run_driver(d, {"LD_LIBRARY_PATH": ABI})
+    assert p.returncode != 0 and "drag is not set" in p.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+@pytest.mark.parametrize("which", ["C1", "C4"])
+def test_driver_coupled_on_gpu(tmp_path, oracle, which, mode):
+    """C1 and the reduced C4 through the Fortran host on the GPU against the oracle, 1e-12."""
+    print(f"case fortran driver coupled {which} {mode}", flush=True)
+    case, nodes = c1_case() if which == "C1" else c4_reduced()
+    d = str(tmp_path / "case")
+    write_coupled(d, case, nodes, mode)
+    p = run_driver(d, {}, timeout=300)
+    print(p.stdout[-400:], p.stderr[-400:])
+    assert p.returncode == 0
+    for g, r in zip(field_of(d), oracle_reference(oracle, case)):
+        assert rel_l2(g, r) <= 1e-12

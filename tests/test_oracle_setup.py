@@ -143,3 +143,20 @@ def test_rk3_drag_relaxation(oracle):
     expect = 5.0 * (1 - h + h * h / 2 - h ** 3 / 6)
     # interior cells away from the mirror ends
     assert np.allclose(q[1, 4:12, 0, 0], expect, rtol=1e-9)
+
+
+def test_lazy_blocks_upload_as_eager(oracle):
+    """A case whose blocks are made as the Runner uploads them (LazyBlock, for the
+    full combined configuration) runs as the same case built at once."""
+    from ice_amd import cases
+    kw = dict(ntree=5, npairs=1, emin=5, emax=9, pair_n=6, steps=2)
+    eager = cases.case_combined(**kw)
+    lazy = cases.case_combined(lazy=True, **kw)
+    assert any(isinstance(B, cases.LazyBlock) for B in lazy.blocks)
+    ra = cases.Runner(oracle, eager, "orc_")
+    rb = cases.Runner(oracle, lazy, "orc_", workers=3)
+    ra.step(2)
+    rb.step(2)
+    for b in range(1, len(eager.blocks) + 1):
+        for p in (1, 2):
+            assert np.array_equal(ra.state(b, p), rb.state(b, p))
