@@ -27,6 +27,12 @@ struct FamDev {
   int cur = 0, old = 0;
   double *beta = nullptr, *dt = nullptr, *tau = nullptr, *src = nullptr, *R = nullptr, *T = nullptr;
   double* imports = nullptr;  // rates in force a coupler set (add_births), [nc][N], or null
+  // the queue of the cells whose interface solve a stage defers (Langmuir-Knudsen families)
+  unsigned* lkq_n = nullptr;
+  long long* lkq_g = nullptr;
+  int* lkq_b = nullptr;
+  double* lkq_r = nullptr;
+  int lkq_cap = 0;
   double *bsum = nullptr, *connF = nullptr;
   uint8_t* inj = nullptr;
   int* bidx = nullptr;
@@ -90,6 +96,7 @@ struct ice_gpu_ctx {
   std::vector<std::vector<int>> bfirst_h, brec_h;  // per family, 1-based (collect_boundary_cells)
   DevStatus* d_st = nullptr;
   int* h_flags = nullptr;    // pinned mirror of the status's bad_state and bad_recon
+  int lk_grid = 1024;        // workgroups of the deferred interface solves' kernel (4 per CU)
   int iter = 0;
   int keep_source = 0;
   int kernel_timing = 0;
@@ -370,6 +377,7 @@ FamArgs fam_args(ice_gpu_ctx* c, int p) {
   a.prim_w = F.buf[F.cur];
   a.beta = F.beta; a.dt = F.dt; a.tau = F.tau; a.inj = F.inj; a.src = F.src; a.R = F.R; a.T = F.T;
   a.imports = F.imports;
+  a.lkq_n = F.lkq_n; a.lkq_g = F.lkq_g; a.lkq_b = F.lkq_b; a.lkq_r = F.lkq_r; a.lkq_cap = F.lkq_cap;
   a.bsum = F.bsum; a.bidx = F.bidx; a.connF = F.connF;
   a.rec = c->d_rec; a.donor = c->d_donor; a.weight = c->d_weight;
   a.bfirst = F.bfirst; a.brec = F.brec; a.bcell_g = F.bcell_g; a.nbcell = F.nbcell;
@@ -519,6 +527,7 @@ void dispatch_stage(ice_gpu_ctx* c, FamArgs& a) {
   L.irs = c->cfg.irs; L.limiter = c->cfg.limiter; L.ndir = c->cfg.ndir;
   L.irs_wg = c->d_irs_wg; L.irs_nwg = c->irs_nwg;
   L.timing = c->kernel_timing; L.kev = &c->kev;
+  L.lk_grid = c->lk_grid;
   int cl = c->fams[a.fam].closure;
   if (cl == ICE_MK) launch_stage_mk(L, a);
   else if (cl == ICE_IG) launch_stage_ig(L, a);
@@ -611,6 +620,11 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return fail(c, "hipSetDevice(%d): %s", device, hipGetErrorString(e));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  {
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    c->lk_grid = std::max(1, 4 * prop.multiProcessorCount);
+  }
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   long long off = 0;
@@ -669,6 +683,17 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_of(N)), dim3(kThreads), 0, c->stream, F.bidx, N, -1);
     if (cfg->irs) {
       if (dalloc(c, &F.R, (size_t)F.nc * N) || dalloc(c, &F.T, (size_t)F.nc * N)) return 1;
+    }
+    if (cfg->coupled && fams[p].evap_select != 0 && fams[p].intf_select == 1) {
+      // room for every cell of a grid up to 16 M cells, for 45 % of a larger one (a cell
+      // past the capacity finishes on its tile)
+      long long cells = 0;
+      for (const BlkDev& B : c->blk) cells += B.cells;
+      long long cap = cells <= (16ll << 20) ? cells : std::max(16ll << 20, cells * 45 / 100);
+      F.lkq_cap = (int)std::min<long long>(cap, (1ll << 31) - 1);
+      if (dalloc(c, &F.lkq_n, 1) || dalloc(c, &F.lkq_g, F.lkq_cap) || dalloc(c, &F.lkq_b, F.lkq_cap) ||
+          dalloc(c, &F.lkq_r, (size_t)F.nc * F.lkq_cap))
+        return 1;
     }
   }
   if (dalloc(c, &c->d_st, 1)) return 1;
@@ -1097,10 +1122,7 @@ int ice_gpu_set_tiles(ice_gpu_ctx* c, int tx, int ty, int tz) {
     if (bad)
       return fail(c, "[ICE::set_tiles] a %d x %d x %d tile has more cells or faces than the %d threads of a workgroup",
                   tx, ty, tz, kThreads);
-    for (int d = 0; d < 3; ++d)
-      if (n[d] > 1 && t[0] * t[1] * t[2] / t[d] * (t[d] + 4) > kMaxStageBox)
-        return fail(c, "[ICE::set_tiles] a %d x %d x %d tile's stencil box along direction %d holds more than %d cells",
-                    tx, ty, tz, d + 1, kMaxStageBox);
+
   }
   c->tile[0] = tx; c->tile[1] = ty; c->tile[2] = tz;
   batched_destroy(c);  // the batched plan's work items are tiles
@@ -1563,6 +1585,7 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
   for (auto& f : c->fam) {
     for (int m = 0; m < 3; ++m) F(f.buf[m]);
     F(f.beta); F(f.dt); F(f.tau); F(f.src); F(f.R); F(f.T); F(f.bsum); F(f.connF); F(f.inj); F(f.bidx); F(f.imports);
+    F(f.lkq_n); F(f.lkq_g); F(f.lkq_b); F(f.lkq_r);
     F(f.grp); F(f.own); F(f.bfirst); F(f.brec); F(f.bcell_g); F(f.prop);
   }
   for (auto& S : c->surr) {
@@ -1742,6 +1765,7 @@ BatchLaunch group_launch(ice_gpu_ctx* c, const BatchGroup& g, const std::vector<
   B.sd = c->cfg.shock_detector;
   B.coupled = c->cfg.coupled;
   B.limiter = c->cfg.limiter;
+  B.lk_grid = c->lk_grid;
   return B;
 }
 

@@ -67,6 +67,14 @@ struct FamArgs {
   uint8_t* inj;         // [N]
   double* src;          // [nc][N] kept source (optional)
   const double* imports;  // [nc][N] rates in force a coupler set (births, absorptions), or null
+  // cells whose Langmuir-Knudsen interface solve is deferred to k_lk_finish (full
+  // waves of such cells): count, padded index, block and flux sum [nc][cap] of each;
+  // null when the family has no such solve
+  unsigned* lkq_n;
+  long long* lkq_g;
+  int* lkq_b;
+  double* lkq_r;
+  int lkq_cap;
   double* R;            // [nc][N] materialised residual (IRS)
   double* T;            // [nc][N] IRS buffer: the directions ping-pong between R and T
   double* bsum;         // [nbcell][nc] boundary-flux sums

@@ -24,7 +24,17 @@ struct StageLaunch {
   int irs_nwg;              // workgroups of that launch
   int timing;                        // bracket each stage launch with events
   std::vector<hipEvent_t>* kev;
+  int lk_grid;                       // workgroups of k_lk_finish (its grid-stride loop)
 };
+
+// The queue of deferred interface solves: cleared before a stage's kernels, then
+// drained by k_lk_finish after them
+inline void clear_lk_queue(const FamArgs& a, hipStream_t stream) {
+  if (a.lkq_g) hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, stream, a.lkq_n);
+}
+template <int M, bool WRITE_R> void launch_lk_finish(const FamArgs& a, int grid, hipStream_t stream) {
+  if (a.lkq_g) hipLaunchKernelGGL((k_lk_finish<M, WRITE_R>), dim3((unsigned)grid), dim3(kThreads), 0, stream, a);
+}
 
 inline unsigned grid_cells(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
@@ -78,6 +88,7 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
     hipLaunchKernelGGL((k_conn<M, S, SD>), dim3(grid_cells(L.nown)), dim3(kThreads), 0, L.stream, a);
   }
   if (L.nbcell) hipLaunchKernelGGL((k_bcell<M, S, SD>), dim3(grid_cells(L.nbcell)), dim3(kThreads), 0, L.stream, a);
+  if (CP) clear_lk_queue(a, L.stream);
   for (int b = 0; b < nb; ++b) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool timed = L.timing && !L.irs;
@@ -96,6 +107,23 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
     } else {
       launch_tiled<M, S, SD, CP>(L, a, b);
     }
+    if (timed) {
+      (void)hipEventRecord(e1, L.stream);
+      L.kev->push_back(e0);
+      L.kev->push_back(e1);
+    }
+  }
+  if (CP && a.lkq_g) {
+    // bracketed as one more launch of the stage's kernels when timing
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool timed = L.timing && !L.irs;
+    if (timed) {
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      (void)hipEventRecord(e0, L.stream);
+    }
+    if (L.irs) launch_lk_finish<M, true>(a, L.lk_grid, L.stream);
+    else launch_lk_finish<M, false>(a, L.lk_grid, L.stream);
     if (timed) {
       (void)hipEventRecord(e1, L.stream);
       L.kev->push_back(e0);
@@ -156,8 +184,19 @@ struct BatchLaunch {
   int irs_nwg;
   const BlkDev* blk;           // host copies of the blocks
   int riemann, sd, coupled, limiter;
+  int lk_grid;                 // workgroups of k_lk_finish
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // recorded around the persistent stage kernel when set
 };
+
+// the deferred interface solves of the group's families, after the stage kernel
+template <int M, bool CP> void finish_lk_group(const BatchLaunch& B, bool write_r) {
+  if (!CP) return;
+  for (int f = 0; f < B.nfam; ++f) {
+    const FamArgs& a = B.h_fams[B.fam_ids[f]];
+    if (write_r) launch_lk_finish<M, true>(a, B.lk_grid, B.stream);
+    else launch_lk_finish<M, false>(a, B.lk_grid, B.stream);
+  }
+}
 
 inline void reset_tickets(const BatchLaunch& B) {
   hipLaunchKernelGGL(k_zero_tickets, dim3(1), dim3(64), 0, B.stream, B.Q.ticket, B.Q.nq);
@@ -189,6 +228,8 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
     if (L.nbcell) hipLaunchKernelGGL((k_bcell<M, S, SD>), dim3(grid_cells(L.nbcell)), dim3(kThreads), 0, B.stream, a);
   }
   reset_tickets(B);
+  if (CP)
+    for (int f = 0; f < B.nfam; ++f) clear_lk_queue(B.h_fams[B.fam_ids[f]], B.stream);
   if (B.ev0) (void)hipEventRecord(B.ev0, B.stream);
   dim3 blk(kThreads);
   if (B.irs) {
@@ -198,6 +239,7 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
                          B.Q.items);
     else
       hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, true, -1>), dim3(B.grid), blk, 0, B.stream, B.d_fams, B.Q);
+    finish_lk_group<M, CP>(B, true);
     if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
     constexpr int NC = Closure<M>::NC;
     for (int f = 0; f < B.nfam; ++f) {
@@ -218,6 +260,7 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
     else if (B.limiter == 3) hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q);
     else hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, -1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
   }
+  finish_lk_group<M, CP>(B, false);
   if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
 }
 

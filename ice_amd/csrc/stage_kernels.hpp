@@ -584,6 +584,96 @@ ICE_DEV void update_cell(const FamArgs& a, const BlkDev& B, int i, int j, int k,
   }
 }
 
+// The end of one cell's stage from its flux sum r: the source (coupled, seeded by
+// seed), the residual (Lib_Residual.f90:147-156), the imports (add_births) and the
+// frozen slots, then the RK update, or the residual for smoothing (WRITE_R).
+template <int M, bool TAB, bool COUPLED, bool WRITE_R>
+ICE_DEV void finish_cell(const FamArgs& a, const BlkDev& Bref, int ci, int cj, int ck, long long g, double* r,
+                         float seed) {
+  constexpr int NC = Closure<M>::NC;
+  double src[NC];
+  if constexpr (COUPLED) {
+    double q[NC], tau;
+    load_cell<NC>(a.cur, a.N, g, q);
+    cell_source<M, TAB>(a, g, q, src, tau, seed);
+    for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
+    keep_stage_source<NC>(a, g, src, tau);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NC; ++v) src[v] = 0.0;
+  }
+  double vol = a.vol[g];
+  double dt = a.time_accurate ? a.st->dtglobal : a.dt[g];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) r[v] = (r[v] / vol + src[v]) * dt;
+  add_imports<NC>(a, g, dt, r);
+  for (int f = 0; f < a.nfrozen; ++f) r[a.frozen[f] - 1] = 0.0;
+  if constexpr (WRITE_R) {
+    store_cell<NC>(a.R, a.N, g, r);
+  } else {
+    update_cell<M, TAB>(a, Bref, ci, cj, ck, g, r);
+  }
+}
+
+// Whether the cell's source runs the Langmuir-Knudsen interface solve (lk_needed)
+template <int M, bool TAB>
+ICE_DEV bool lk_cell_needed(const FamArgs& a, long long g) {
+  constexpr int NC = Closure<M>::NC;
+  const double Tp = a.cur[(long long)(NC - 2) * a.N + g];
+  double ep[kNepCell];
+  cell_ep<TAB>(a, g, Tp, ep);
+  double p;
+  return lk_needed(a.gas[g], a.gas[4 * a.N + g], a.gR[g], Tp, a.mat.evap, a.mat.intf, ep, p);
+}
+
+// Appends the wave's cells that need the interface solve to the family's queue (one
+// atomic per wave), with their block and flux sum; false for the others, and for
+// every cell past the queue's capacity (those finish where they are)
+template <int NC>
+ICE_DEV bool defer_cell(const FamArgs& a, bool need, long long g, int b, const double* r) {
+  const unsigned long long mask = __ballot(need);
+  if (mask == 0ull) return false;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned base = 0u;
+  if (lane == leader) base = atomicAdd(a.lkq_n, (unsigned)__popcll(mask));
+  base = (unsigned)__shfl((int)base, leader, 64);
+  if (!need) return false;
+  const unsigned slot = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+  if (slot >= (unsigned)a.lkq_cap) return false;
+  a.lkq_g[slot] = g;
+  a.lkq_b[slot] = b;
+#pragma unroll
+  for (int v = 0; v < NC; ++v) a.lkq_r[(long long)v * a.lkq_cap + slot] = r[v];
+  return true;
+}
+
+// The deferred cells of a stage, in full waves: their source with the interface
+// solve (seeded as on the tile), residual and update, in any order (each cell's
+// result is its own).  A grid-stride loop over the queue; every wave of a workgroup
+// runs the same trips.
+template <int M, bool WRITE_R>
+__global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_lk_finish(FamArgs a) {
+  constexpr int NC = Closure<M>::NC;
+  __shared__ __attribute__((aligned(16))) uint16_t mlds[kMlpLds];
+  const unsigned n0 = *a.lkq_n, n = n0 < (unsigned)a.lkq_cap ? n0 : (unsigned)a.lkq_cap;
+  for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const unsigned idx = base + threadIdx.x;
+    const bool has = idx < n;
+    const long long g = has ? a.lkq_g[idx] : 0ll;
+    const float seed = stage_seed<M, false>(a, has, g, mlds);
+    if (has) {
+      double r[NC];
+#pragma unroll
+      for (int v = 0; v < NC; ++v) r[v] = a.lkq_r[(long long)v * a.lkq_cap + idx];
+      const BlkDev& B = a.blk[a.lkq_b[idx]];
+      const long long loc = g - B.off;
+      const int i = (int)(loc % B.NI) - 1, j = (int)((loc / B.NI) % B.NJ) - 1, k = (int)(loc / ((long long)B.NI * B.NJ)) - 1;
+      finish_cell<M, false, true, WRITE_R>(a, B, i, j, k, g, r, seed);
+    }
+  }
+}
+
 // The cell's stage: boundary sum, interior faces in the reference's order
 // (Mod_Fluxes.f90:471-508), source, residual (Lib_Residual.f90:147-156) and
 // either the RK update (WRITE_R = false) or the residual for smoothing.
@@ -749,44 +839,6 @@ struct Tile {
 
 // slots of the face fluxes of one direction in LDS, structure of arrays
 constexpr int kMaxTileFaces = kThreads;
-// cells of the stencil box of one direction of a tile: the tile's cells widened
-// by two along the direction, t_d + 4 by the other two extents
-constexpr int kMaxStageBox = 512;
-// doubles of the LDS stencil box of a closure (its nc slots and the cells'
-// lengths along the direction), shared with the learned seed's buffer, which is
-// used only after the faces
-template <int NC> constexpr int stage_lds_doubles() {
-  return (NC + 1) * kMaxStageBox > kMlpLds / 4 ? (NC + 1) * kMaxStageBox : kMlpLds / 4;
-}
-
-// face_acc (MODE 2) with the face's four stencil cells and their lengths along d
-// read from the tile's LDS box: box[v * nbox + c] holds slot v of box cell c, slot
-// NC the cell's length along d; c is the box cell of the face's lower cell, bstride
-// the box's stride along d.  The same operations as face_acc on the same values.
-template <int M, int S, bool SD, int LIM, bool TAB>
-ICE_DEV bool face_box(const FamArgs& a, long long g, long long stride, int d, int m, int n, const double* box,
-                      int nbox, int c, int bstride, double* r) {
-  constexpr int NC = Closure<M>::NC;
-  const long long N = a.N;
-  const double* len = box + NC * nbox;
-  double len0 = len[c - bstride], len1 = len[c], len2 = len[c + bstride], len3 = len[c + 2 * bstride];
-  double bf = SD ? dmin2(a.beta[g], a.beta[g + stride]) : 1.0;
-  if (m == 1 && a.inj[g - stride]) len0 = 0.0;
-  if (m == n - 1 && a.inj[g + 2 * stride]) len3 = 0.0;
-  const double* fa = a.fa + (long long)(d * 4) * N;
-  double nrm[3] = {fa[g], fa[N + g], fa[2 * N + g]};
-  double area = fa[3 * N + g];
-  double dl0 = 0.5 * (len0 + len1);
-  double dl1 = 0.5 * (len1 + len2);
-  double dl2 = 0.5 * (len2 + len3);
-  double dll = 0.5 * len1;
-  double dlr = 0.5 * len2;
-  double ql[NC], qr[NC];
-  bool ok = reconstruct_mem<M, LIM>(a.limiter, box + (c - bstride), bstride, nbox, dl0, dl1, dl2, dll, dlr, bf, ql, qr);
-  riemann_acc<M, S, 2>(ql, qr, nrm, area, r, mat_of<TAB>(a));
-  return ok;
-}
-
 // The stage with the interior faces shared through LDS (Mod_Fluxes.f90:421-521:
 // each face once, each cell its two faces per direction in the odd-then-even
 // order), then the cell's source, residual and update as in k_stage.
@@ -795,8 +847,7 @@ ICE_DEV bool face_box(const FamArgs& a, long long g, long long stride, int d, in
 // LDS buffer of face fluxes.
 // TAB = false: the launch serves only families without tabulated properties
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM, bool TAB = false>
-ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl, double* box,
-                             uint16_t* mlds) {
+ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl, uint16_t* mlds) {
   constexpr int NC = Closure<M>::NC;
   const BlkDev& Bref = a.blk[b];
   const BlkDev B = Bref;
@@ -828,39 +879,15 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
     int fx = T.tx + (d == 0), fy = T.ty + (d == 1), fz = T.tz + (d == 2);
     int nf = fx * fy * fz;
     long long stride = dir_stride(B, d);
-    // the stencil box: the tile's cells widened by two along d (offsets -2 .. t_d+1),
-    // each cell's nc slots and its length along d, loaded once from HBM/L2; cells
-    // past the block's ghost layers or its other extents are not loaded (no face
-    // reads them)
-    const int bx = T.tx + 4 * (d == 0), by = T.ty + 4 * (d == 1), bz = T.tz + 4 * (d == 2);
-    const int nbox = bx * by * bz;
-    const int x0 = ox - 2 * (d == 0), y0 = oy - 2 * (d == 1), z0 = oz - 2 * (d == 2);
-    const int xm = B.ni + 2 * (d == 0), ym = B.nj + 2 * (d == 1), zm = B.nk + 2 * (d == 2);
-    const double* dlen = a.dl + (long long)d * a.N;
-    for (int c = t; c < nbox; c += blockDim.x) {
-      int bi = c % bx, bj = (c / bx) % by, bk = c / (bx * by);
-      int i = x0 + bi, j = y0 + bj, k = z0 + bk;
-      if (i <= xm && j <= ym && k <= zm) {
-        long long gg = gidx(B, i, j, k);
-#pragma unroll
-        for (int v = 0; v < NC; ++v) box[v * nbox + c] = a.cur[(long long)v * a.N + gg];
-        box[NC * nbox + c] = dlen[gg];
-      }
-    }
-    const int bstride = d == 0 ? 1 : (d == 1 ? bx : bx * by);
-    __syncthreads();
     for (int f = t; f < nf; f += blockDim.x) {
-      int fi = f % fx, fj = (f / fx) % fy, fk = f / (fx * fy);
-      int li = ox + fi - (d == 0);
-      int lj = oy + fj - (d == 1);
-      int lk = oz + fk - (d == 2);
+      int li = ox + f % fx - (d == 0);
+      int lj = oy + (f / fx) % fy - (d == 1);
+      int lk = oz + f / (fx * fy) - (d == 2);
       int m = d == 0 ? li : (d == 1 ? lj : lk);
       bool inside = m >= 1 && m <= n - 1 && li >= 1 && lj >= 1 && lk >= 1 && li <= B.ni && lj <= B.nj && lk <= B.nk;
       if (inside) {
-        // the box cell of the face's lower cell: offset +1 along d from the face grid
-        int c = (fi + (d == 0)) + bx * ((fj + (d == 1)) + by * (fk + (d == 2)));
         double q[NC];
-        ok = face_box<M, S, SD, LIM, TAB>(a, gidx(B, li, lj, lk), stride, d, m, n, box, nbox, c, bstride, q) && ok;
+        ok = face_acc<M, S, SD, 2, LIM, TAB>(a, gidx(B, li, lj, lk), stride, d, m, n, q) && ok;
 #pragma unroll
         for (int v = 0; v < NC; ++v) fl[v * kMaxTileFaces + f] = q[v];
       }
@@ -893,31 +920,16 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
     __syncthreads();
   }
   if (!ok) atomicOr(&a.st->bad_recon, 1);
-  float seed = 0.0f;
-  if constexpr (COUPLED) seed = stage_seed<M, TAB>(a, has_cell, g, mlds);
-  if (!has_cell) return;
-  double src[NC];
+  // a cell whose source runs the Langmuir-Knudsen solve finishes in k_lk_finish,
+  // where such cells fill whole waves instead of holding up the rest of theirs
+  bool deferred = false;
   if constexpr (COUPLED) {
-    double q[NC], tau;
-    load_cell<NC>(a.cur, a.N, g, q);
-    cell_source<M, TAB>(a, g, q, src, tau, seed);
-    for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
-    keep_stage_source<NC>(a, g, src, tau);
-  } else {
-#pragma unroll
-    for (int v = 0; v < NC; ++v) src[v] = 0.0;
+    if (a.lkq_g) deferred = defer_cell<NC>(a, has_cell && lk_cell_needed<M, TAB>(a, g), g, b, r);
   }
-  double vol = a.vol[g];
-  double dt = a.time_accurate ? a.st->dtglobal : a.dt[g];
-#pragma unroll
-  for (int v = 0; v < NC; ++v) r[v] = (r[v] / vol + src[v]) * dt;
-  add_imports<NC>(a, g, dt, r);
-  for (int f = 0; f < a.nfrozen; ++f) r[a.frozen[f] - 1] = 0.0;
-  if constexpr (WRITE_R) {
-    store_cell<NC>(a.R, a.N, g, r);
-  } else {
-    update_cell<M, TAB>(a, Bref, ci, cj, ck, g, r);
-  }
+  float seed = 0.0f;
+  if constexpr (COUPLED) seed = stage_seed<M, TAB>(a, has_cell && !deferred, g, mlds);
+  if (!has_cell || deferred) return;
+  finish_cell<M, TAB, COUPLED, WRITE_R>(a, Bref, ci, cj, ck, g, r, seed);
 }
 
 // Workgroup w of a launch runs on XCD w % 8 (workgroups are dealt round-robin
@@ -936,8 +948,8 @@ template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_tiled(FamArgs a, int b, Tile T) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
-  __shared__ __attribute__((aligned(16))) double box[stage_lds_doubles<NC>()];
-  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(a, b, T, xcd_tile(blockIdx.x, gridDim.x), fl, box, (uint16_t*)box);
+  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
+  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(a, b, T, xcd_tile(blockIdx.x, gridDim.x), fl, mlds);
 }
 
 // ------------------------------------------------------------------ batched path
@@ -977,14 +989,14 @@ template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_persist(const FamArgs* fams, WorkQueues Q) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
-  __shared__ __attribute__((aligned(16))) double box[stage_lds_doubles<NC>()];
+  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
   __shared__ int slot;
   int q = blockIdx.x % Q.nq;
   for (;;) {
     int it = next_item(Q, q, &slot);
     if (it < 0) break;
     const WorkItem w = Q.items[it];
-    stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, box, (uint16_t*)box);
+    stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
   }
 }
 
@@ -997,10 +1009,10 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_items(
                                                                                  const WorkItem* items) {
   constexpr int NC = Closure<M>::NC;
   __shared__ double fl[NC * kMaxTileFaces];
-  __shared__ __attribute__((aligned(16))) double box[stage_lds_doubles<NC>()];
+  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
   const int it = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x));
   const WorkItem w = items[it];
-  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, box, (uint16_t*)box);
+  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
 }
 
 // the sensor weights and the dt pass over the same queues, one cell per thread
@@ -1056,6 +1068,9 @@ __global__ void __launch_bounds__(kThreads) k_dt_persist(const FamArgs* fams, Wo
 namespace {  // one copy per translation unit
 __global__ void k_zero_tickets(unsigned long long* t, int n) {
   if (threadIdx.x < n) t[threadIdx.x] = 0ull;
+}
+__global__ void k_zero_count(unsigned* n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *n = 0u;
 }
 }  // namespace
 

@@ -493,6 +493,14 @@ int ice_gpu_surrogate_train(ice_gpu_ctx* c, int p, int epochs, int batch, double
   return surr_train_impl(c, p, epochs, batch, lr, seed, report, false);
 }
 
+int ice_gpu_surrogate_retrain(ice_gpu_ctx* c, int p, int epochs) {
+  if (p < 1 || p > c->cfg.ngroups) return fail(c, "[ICE::surrogate_retrain] no family %d", p);
+  if (epochs < 1) return fail(c, "[ICE::surrogate_retrain] epochs >= 1");
+  SurrDev& S = c->surr[p - 1];
+  if (!S.on) return fail(c, "[ICE::surrogate_retrain] family %d has no trained network", p);
+  return surr_train_impl(c, p, epochs, S.batch, S.lr, S.seed, nullptr, true);
+}
+
 int ice_gpu_surrogate_retrain_every(ice_gpu_ctx* c, int every, int epochs) {
   if (every < 0 || epochs < 1) return fail(c, "[ICE::surrogate_retrain_every] every >= 0 and epochs >= 1");
   c->retrain_every = every;

@@ -99,6 +99,7 @@ SIGNATURES = {
     "ice_gpu_mlp_forward": [C.c_void_p, C.c_int, _FP, _FP, _FP],
     "ice_gpu_surrogate_train": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, _FP],
     "ice_gpu_surrogate_retrain_every": [C.c_void_p, C.c_int, C.c_int],
+    "ice_gpu_surrogate_retrain": [C.c_void_p, C.c_int, C.c_int],
     "ice_gpu_surrogate_params": [C.c_void_p, C.c_int, _FP],
     "ice_gpu_surrogate_set_params": [C.c_void_p, C.c_int, _FP],
     "ice_gpu_set_lk_stats": [C.c_void_p, C.c_int],

@@ -133,3 +133,17 @@ def table_value(tab: Table, c: int, T: float) -> float:
         return float(t[-1])
     i = int(Tc)
     return float(t[i - lo] + (t[i + 1 - lo] - t[i - lo]) * (Tc - i))
+
+
+def table_e(tab: Table, T):
+    """The table's energy e(T) (mat_e, Lib_Properties.f90:163-200) on an array of
+    temperatures: linear between the nodes, the line through e(0) = 0 below Tmin
+    (the first segment when Tmin = 0), the last segment's slope above Tmax."""
+    T = np.asarray(T, float)
+    t = tab.col(E)
+    lo, hi = tab.t_min, tab.t_min + tab.n - 1
+    i = np.clip(np.floor(T).astype(np.int64), lo, hi - 1)
+    mid = t[i - lo] + (t[i + 1 - lo] - t[i - lo]) * (T - i)
+    below = t[0] * (T / lo) if lo > 0 else (t[1] - t[0]) * T
+    above = t[-1] + (t[-1] - t[-2]) * (T - hi)
+    return np.where(T < lo, below, np.where(T >= hi, above, mid))

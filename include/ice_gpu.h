@@ -217,6 +217,9 @@ int  ice_gpu_surrogate_train(ice_gpu_ctx* ctx, int p, int epochs, int batch, dou
  * multiple of it), each trained network is trained again, `epochs` passes from its
  * current weights, on samples of the current state (0 turns it off). */
 int  ice_gpu_surrogate_retrain_every(ice_gpu_ctx* ctx, int every, int epochs);
+/* The same retraining of family p's network, now: `epochs` passes from its current
+ * weights on samples of the current state, with the settings of its last training. */
+int  ice_gpu_surrogate_retrain(ice_gpu_ctx* ctx, int p, int epochs);
 int  ice_gpu_surrogate_params(ice_gpu_ctx* ctx, int p, float* params);
 int  ice_gpu_surrogate_set_params(ice_gpu_ctx* ctx, int p, const float* params);
 /* 1: count the interface solves of every stage; ice_gpu_lk_stats: family p's solves,

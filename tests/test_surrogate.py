@@ -181,3 +181,43 @@ def test_trained_seed_on_evaporation_case(gpu):
     print("map evaluations per solve: plain %.2f, secant from equilibrium %.2f, network seed %.2f" %
           (per[0], per[2], per[1]))
     assert per[1] < per[2] < per[0]
+
+
+def retrain_run(gpu, every, explicit, steps=6, n=32, epochs=5):
+    """The evaporation case with the network trained at step 0 and retrained every
+    `every` steps, by ice_gpu_step itself (explicit False) or by explicit calls of
+    ice_gpu_surrogate_retrain after the same steps: (state, network parameters)."""
+    case = cases.case_evaporation(n=n, steps=steps)
+    case.fams[0].surrogate = 1
+    r = cases.Runner(gpu, case, "ice_gpu_")
+    rep = np.zeros(6, np.float32)
+    assert gpu.ice_gpu_surrogate_train(r.ctx, 1, 50, 256, 3e-3, 11, rep) == 0, r.error()
+    p0 = np.zeros(P, np.float32)
+    assert gpu.ice_gpu_surrogate_params(r.ctx, 1, p0) == 0
+    if explicit:
+        for _ in range(steps // every):
+            r.step(every)
+            assert gpu.ice_gpu_surrogate_retrain(r.ctx, 1, epochs) == 0, r.error()
+    else:
+        assert gpu.ice_gpu_surrogate_retrain_every(r.ctx, every, epochs) == 0
+        r.step(steps)
+    p1 = np.zeros(P, np.float32)
+    assert gpu.ice_gpu_surrogate_params(r.ctx, 1, p1) == 0
+    s = r.state(1, 1)
+    r.close()
+    return s, p0, p1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("every", [2, 6])
+def test_online_retraining(gpu, every):
+    """Online retraining every 2 and every 6 steps gives the network and the state
+    that the same retraining done by explicit calls after the same steps gives, bit
+    for bit; retraining moves the weights."""
+    print("case evaporation 32^3 retrain every", every, flush=True)
+    s_a, p0_a, p1_a = retrain_run(gpu, every, explicit=False)
+    s_b, p0_b, p1_b = retrain_run(gpu, every, explicit=True)
+    assert np.array_equal(p0_a, p0_b)
+    assert not np.array_equal(p1_a, p0_a)
+    assert np.array_equal(p1_a, p1_b)
+    assert np.array_equal(s_a, s_b)
