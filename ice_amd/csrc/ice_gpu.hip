@@ -148,6 +148,9 @@ template <typename T> int dalloc(ice_gpu_ctx* c, T** p, size_t n) {
 
 int nc_of(int closure) { return closure == ICE_MK ? 6 : (closure == ICE_IG ? 7 : 12); }
 
+// the scratch array of the entry point that allocates nothing, on the context's device
+int noalloc_setup(ice_gpu_ctx* c);
+
 long long padded_of(const BlkDev& B, int i, int j, int k) {
   return B.off + (long long)(i + 1) + (long long)B.NI * ((long long)(j + 1) + (long long)B.NJ * (long long)(k + 1));
 }
@@ -619,6 +622,7 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
   *out = c;
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return fail(c, "hipSetDevice(%d): %s", device, hipGetErrorString(e));
+  if (noalloc_setup(c)) return 1;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   {
     hipDeviceProp_t prop;
@@ -1863,3 +1867,5 @@ void batched_destroy(ice_gpu_ctx* c) {
 
 // the learned seed of the interface solve (same translation unit: it uses the context)
 #include "surrogate.hpp"
+// the residual norm without allocation (same translation unit: it uses the context)
+#include "noalloc.hpp"

@@ -68,7 +68,8 @@ module ice_gpu_m
             ice_gpu_mlp_grad, ice_gpu_mlp_forward, ice_gpu_surrogate_train, ice_gpu_surrogate_retrain_every, &
             ice_gpu_surrogate_params, ice_gpu_surrogate_set_params, ice_gpu_set_lk_stats, ice_gpu_lk_stats, &
             ice_gpu_set_mode, ice_gpu_set_surrogate, ice_gpu_set_counters, ice_gpu_get_counters, &
-            ice_gpu_fill_ghosts, ice_gpu_put_imports, ice_gpu_add_births, ice_gpu_get_tau, ice_gpu_surrogate_retrain
+            ice_gpu_fill_ghosts, ice_gpu_put_imports, ice_gpu_add_births, ice_gpu_get_tau, ice_gpu_surrogate_retrain, &
+            ice_gpu_residual_norm_noalloc
   integer, parameter, public :: ICE_MLP_PARAMS = 6288
 
   interface
@@ -272,6 +273,11 @@ module ice_gpu_m
       type(c_ptr), value :: ctx
       integer(c_int), value :: p
       integer(c_int64_t) :: counts(4)
+    end function
+    integer(c_int) function ice_gpu_residual_norm_noalloc(ctx, total, fam) bind(C, name='ice_gpu_residual_norm_noalloc')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: total(5), fam(*)
     end function
     integer(c_int) function ice_gpu_surrogate_retrain(ctx, p, epochs) bind(C, name='ice_gpu_surrogate_retrain')
       import :: c_int, c_ptr

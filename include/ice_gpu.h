@@ -169,6 +169,14 @@ int  ice_gpu_get_source(ice_gpu_ctx* ctx, int b, int p, double* source, double* 
 /* residual_norm (Mod_Diagnostic.f90:22-69) of the last step: total[5], fam[5*ngroups]. */
 int  ice_gpu_residual_norm(ice_gpu_ctx* ctx, double* total, double* fam);
 
+/* residual_norm of the last step as ice_gpu_residual_norm, with no allocation: no
+ * counterpart in the reference; for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its device memory is a file-scope array of 1.5 MiB per GPU, which
+ * holds 604 block-family pairs up to 8 families (larger grids get an error: use
+ * ice_gpu_residual_norm); calls that overlap on one GPU race. */
+int  ice_gpu_residual_norm_noalloc(ice_gpu_ctx* ctx, double* total, double* fam);
+
 /* Time (ms, HIP events on the context's stream) of the last ice_gpu_step, and
  * the summed duration and launch count of the dominant kernel in it. */
 int  ice_gpu_timing(ice_gpu_ctx* ctx, double* step_ms, double* kernel_ms, int64_t* kernel_launches);
