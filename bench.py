@@ -250,6 +250,9 @@ def main():
                     help="the learned seed of the interface solve (auto: on where the workload has one)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default stage tile (otherwise the library's autotuner picks it, untimed)")
+    ap.add_argument("--tile", default=None,
+                    help="stage tile tx,ty,tz (with --dispatch): set instead of autotuned, as for a profile")
+    ap.add_argument("--dispatch", type=int, default=None, help="batched dispatch with --tile: 0 queues, 1 items")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB three times on the device (the PMC traffic calibration kernel) and exit")
     ap.add_argument("--cpu-steps", type=int, default=50,
@@ -283,7 +286,9 @@ def main():
 
     gpu = native.load()
     tuned = None
-    if not args.no_autotune and args.workload in TUNE_PROXY:
+    if args.tile:
+        tuned = {"tile": [int(x) for x in args.tile.split(",")], "dispatch": args.dispatch or 0, "on": "given"}
+    elif not args.no_autotune and args.workload in TUNE_PROXY:
         pcase = WORKLOADS[TUNE_PROXY[args.workload]](mode)
         prun = cases.Runner(gpu, pcase, "ice_gpu_")
         tuned = autotune(gpu, prun)

@@ -83,8 +83,10 @@ struct ice_gpu_ctx {
   std::vector<ice_family_t> fams;
   std::vector<BlkDev> blk;
   BlkDev* d_blk = nullptr;
-  int* d_irs_wg = nullptr;   // first workgroup of each block in the all-blocks IRS launch, then the total
+  int* d_irs_wg = nullptr;   // first workgroup of each block in the all-blocks cell launches, then the total
   int irs_nwg = 0;
+  int* d_irs_wl[2] = {nullptr, nullptr};  // the same over the lines along j and k (marching smoothing)
+  int irs_nwl[2] = {0, 0};
   long long N = 0;
   double *vol = nullptr, *dl = nullptr, *M = nullptr, *fa = nullptr;
   double *gas = nullptr, *gR = nullptr, *gG = nullptr, *gK = nullptr, *gMu = nullptr, *yinf = nullptr;
@@ -474,9 +476,9 @@ __global__ void k_resnorm_sum(const double* __restrict__ part, double* __restric
 
 void launch_dt(ice_gpu_ctx* c, int p, const FamArgs& a) {
   int cl = c->fams[p].closure, nb = (int)c->blk.size();
-  if (cl == ICE_MK) launch_dt_mk(c->stream, c->blk.data(), nb, a);
-  else if (cl == ICE_IG) launch_dt_ig(c->stream, c->blk.data(), nb, a);
-  else launch_dt_ag(c->stream, c->blk.data(), nb, a);
+  if (cl == ICE_MK) launch_dt_mk(c->stream, c->d_irs_wg, c->irs_nwg, nb, a);
+  else if (cl == ICE_IG) launch_dt_ig(c->stream, c->d_irs_wg, c->irs_nwg, nb, a);
+  else launch_dt_ag(c->stream, c->d_irs_wg, c->irs_nwg, nb, a);
 }
 
 // The stage tile of a block: the context's shape, else one whose face grids in
@@ -529,6 +531,7 @@ void dispatch_stage(ice_gpu_ctx* c, FamArgs& a) {
   L.riemann = c->fams[a.fam].riemann; L.sd = c->cfg.shock_detector; L.coupled = c->cfg.coupled;
   L.irs = c->cfg.irs; L.limiter = c->cfg.limiter; L.ndir = c->cfg.ndir;
   L.irs_wg = c->d_irs_wg; L.irs_nwg = c->irs_nwg;
+  for (int d = 0; d < 2; ++d) { L.irs_wl[d] = c->d_irs_wl[d]; L.irs_nwl[d] = c->irs_nwl[d]; }
   L.timing = c->kernel_timing; L.kev = &c->kev;
   L.lk_grid = c->lk_grid;
   int cl = c->fams[a.fam].closure;
@@ -654,6 +657,15 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
     c->irs_nwg = wg[nb];
     if (dalloc(c, &c->d_irs_wg, nb + 1)) return 1;
     HIPCHK(hipMemcpy(c->d_irs_wg, wg.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+    for (int d = 1; d <= 2; ++d) {
+      for (int b = 0; b < nb; ++b) {
+        const BlkDev& B = c->blk[b];
+        wg[b + 1] = wg[b] + (int)grid_cells((long long)B.ni * (d == 1 ? B.nk : B.nj));
+      }
+      c->irs_nwl[d - 1] = wg[nb];
+      if (dalloc(c, &c->d_irs_wl[d - 1], nb + 1)) return 1;
+      HIPCHK(hipMemcpy(c->d_irs_wl[d - 1], wg.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+    }
   }
   if (dalloc(c, &c->vol, N) || dalloc(c, &c->dl, 3 * N) || dalloc(c, &c->M, 9 * N) || dalloc(c, &c->fa, 12 * N))
     return 1;
@@ -1584,7 +1596,7 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
   auto F = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  F(c->d_blk); F(c->d_irs_wg); F(c->vol); F(c->dl); F(c->M); F(c->fa); F(c->gas); F(c->gR); F(c->gG); F(c->gK); F(c->gMu);
+  F(c->d_blk); F(c->d_irs_wg); F(c->d_irs_wl[0]); F(c->d_irs_wl[1]); F(c->vol); F(c->dl); F(c->M); F(c->fa); F(c->gas); F(c->gR); F(c->gG); F(c->gK); F(c->gMu);
   F(c->yinf); F(c->d_rec); F(c->d_donor); F(c->d_weight); F(c->d_st);
   for (auto& f : c->fam) {
     for (int m = 0; m < 3; ++m) F(f.buf[m]);
@@ -1763,6 +1775,7 @@ BatchLaunch group_launch(ice_gpu_ctx* c, const BatchGroup& g, const std::vector<
   B.ndir = c->cfg.ndir;
   B.irs_wg = c->d_irs_wg;
   B.irs_nwg = c->irs_nwg;
+  for (int d = 0; d < 2; ++d) { B.irs_wl[d] = c->d_irs_wl[d]; B.irs_nwl[d] = c->irs_nwl[d]; }
   B.nb = (int)c->blk.size();
   B.blk = c->blk.data();
   B.riemann = g.riemann;

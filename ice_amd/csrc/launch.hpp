@@ -20,8 +20,10 @@ struct StageLaunch {
   int nown;
   int nbcell;
   int riemann, sd, coupled, irs, limiter, ndir;
-  const int* irs_wg;        // first workgroup of each block in the all-blocks IRS launch
-  int irs_nwg;              // workgroups of that launch
+  const int* irs_wg;        // first workgroup of each block in the all-blocks cell launches (IRS along i,
+  int irs_nwg;              // the update, the dt pass), and the workgroups of such a launch
+  const int* irs_wl[2];     // the same for the lines along j and k of the marching smoothing
+  int irs_nwl[2];
   int timing;                        // bracket each stage launch with events
   std::vector<hipEvent_t>* kev;
   int lk_grid;                       // workgroups of k_lk_finish (its grid-stride loop)
@@ -39,14 +41,20 @@ template <int M, bool WRITE_R> void launch_lk_finish(const FamArgs& a, int grid,
 inline unsigned grid_cells(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
 // Implicit residual smoothing of a family over every block: one launch per direction,
-// both Jacobi sweeps fused, from R into T, then T into R, and so on.  Returns the
+// both Jacobi sweeps fused, from R into T, then T into R, and so on: along i one
+// thread per cell, along j and k one per line and slot, marching.  Returns the
 // buffer that holds the smoothed residual.
 template <int NC>
-const double* launch_irs(const FamArgs& a, int ndir, const int* wg, int nwg, int nb, hipStream_t stream) {
+const double* launch_irs(const FamArgs& a, int ndir, const int* wg, int nwg, const int* const* wl, const int* nwl,
+                         int nb, hipStream_t stream) {
   const double* src = a.R;
   double* dst = a.T;
   for (int d = 0; d < ndir; ++d) {
-    hipLaunchKernelGGL(k_irs<NC>, dim3((unsigned)nwg), dim3(kThreads), 0, stream, a, d, src, dst, wg, nb);
+    if (d == 0)
+      hipLaunchKernelGGL(k_irs<NC>, dim3((unsigned)nwg), dim3(kThreads), 0, stream, a, d, src, dst, wg, nb);
+    else
+      hipLaunchKernelGGL(k_irs_march<NC>, dim3((unsigned)nwl[d - 1], NC), dim3(kThreads), 0, stream, a, d, src, dst,
+                         wl[d - 1], nb);
     double* t = const_cast<double*>(src);
     src = dst;
     dst = t;
@@ -59,7 +67,8 @@ void launch_tiled(const StageLaunch& L, FamArgs& a, int b) {
   const Tile T = L.tiles[b];
   dim3 grid((unsigned)(T.ntx * T.nty * T.ntz)), blk(kThreads);
   if (L.irs) {
-    hipLaunchKernelGGL((k_stage_tiled<M, S, SD, CP, true, -1>), grid, blk, 0, L.stream, a, b, T);
+    if (L.limiter == 3) hipLaunchKernelGGL((k_stage_tiled<M, S, SD, CP, true, 3>), grid, blk, 0, L.stream, a, b, T);
+    else hipLaunchKernelGGL((k_stage_tiled<M, S, SD, CP, true, -1>), grid, blk, 0, L.stream, a, b, T);
   } else if (L.limiter == 1) {
     hipLaunchKernelGGL((k_stage_tiled<M, S, SD, CP, false, 1>), grid, blk, 0, L.stream, a, b, T);
   } else if (L.limiter == 3) {
@@ -131,7 +140,7 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
     }
   }
   if (L.irs) {
-    const double* res = launch_irs<NC>(a, L.ndir, L.irs_wg, L.irs_nwg, nb, L.stream);
+    const double* res = launch_irs<NC>(a, L.ndir, L.irs_wg, L.irs_nwg, L.irs_wl, L.irs_nwl, nb, L.stream);
     hipLaunchKernelGGL(k_update<M>, dim3((unsigned)L.irs_nwg), dim3(kThreads), 0, L.stream, a, res, L.irs_wg, nb);
   }
 }
@@ -159,9 +168,9 @@ void launch_births_kernel(hipStream_t stream, const FamArgs& a, const long long*
     hipLaunchKernelGGL(k_add_births<M>, dim3(grid_cells(ncell)), dim3(kThreads), 0, stream, a, cell, first, pk, ncell);
 }
 
-template <int M> void launch_dt_kernel(hipStream_t stream, const BlkDev* blk, int nb, const FamArgs& a) {
-  for (int b = 0; b < nb; ++b)
-    hipLaunchKernelGGL(k_dt<M>, dim3(grid_cells(blk[b].cells)), dim3(kThreads), 0, stream, a, b);
+// the dt pass of a family over every block, one launch (wg: the all-blocks cell table)
+template <int M> void launch_dt_kernel(hipStream_t stream, const int* wg, int nwg, int nb, const FamArgs& a) {
+  hipLaunchKernelGGL(k_dt_blocks<M>, dim3((unsigned)nwg), dim3(kThreads), 0, stream, a, wg, nb);
 }
 
 // ------------------------------------------------------------------ batched path
@@ -180,8 +189,10 @@ struct BatchLaunch {
   int dispatch;                // 0: persistent workgroups over the queues, 1: one workgroup per item
   int nitems;
   int irs, ndir, nb;           // implicit residual smoothing: the stage writes R, then the sweeps
-  const int* irs_wg;           // first workgroup of each block in the all-blocks IRS launch
+  const int* irs_wg;           // first workgroup of each block in the all-blocks cell launches
   int irs_nwg;
+  const int* irs_wl[2];        // the same for the lines along j and k
+  int irs_nwl[2];
   const BlkDev* blk;           // host copies of the blocks
   int riemann, sd, coupled, limiter;
   int lk_grid;                 // workgroups of k_lk_finish
@@ -234,17 +245,21 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
   dim3 blk(kThreads);
   if (B.irs) {
     // residual into R, smoothed per block and direction, then the update
-    if (B.dispatch == 1)
-      hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, true, -1>), dim3((unsigned)B.nitems), blk, 0, B.stream, B.d_fams,
-                         B.Q.items);
+    const dim3 items((unsigned)B.nitems), grid(B.grid);
+    if (B.dispatch == 1 && B.limiter == 3)
+      hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, true, 3>), items, blk, 0, B.stream, B.d_fams, B.Q.items);
+    else if (B.dispatch == 1)
+      hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, true, -1>), items, blk, 0, B.stream, B.d_fams, B.Q.items);
+    else if (B.limiter == 3)
+      hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, true, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q);
     else
-      hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, true, -1>), dim3(B.grid), blk, 0, B.stream, B.d_fams, B.Q);
+      hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, true, -1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
     finish_lk_group<M, CP>(B, true);
     if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
     constexpr int NC = Closure<M>::NC;
     for (int f = 0; f < B.nfam; ++f) {
       FamArgs a = B.h_fams[B.fam_ids[f]];
-      const double* res = launch_irs<NC>(a, B.ndir, B.irs_wg, B.irs_nwg, B.nb, B.stream);
+      const double* res = launch_irs<NC>(a, B.ndir, B.irs_wg, B.irs_nwg, B.irs_wl, B.irs_nwl, B.nb, B.stream);
       hipLaunchKernelGGL(k_update<M>, dim3((unsigned)B.irs_nwg), blk, 0, B.stream, a, res, B.irs_wg, B.nb);
     }
     return;
@@ -271,9 +286,10 @@ template <int M, int S> void launch_batched_sd_cp(const BatchLaunch& B) {
   else launch_batched_stage<M, S, false, false>(B);
 }
 
+// the dt pass of the group's families, one launch each over every block
 template <int M> void launch_batched_dt_kernel(const BatchLaunch& B) {
-  reset_tickets(B);
-  hipLaunchKernelGGL(k_dt_persist<M>, dim3(B.grid), dim3(kThreads), 0, B.stream, B.d_fams, B.Q);
+  for (int f = 0; f < B.nfam; ++f)
+    launch_dt_kernel<M>(B.stream, B.irs_wg, B.irs_nwg, B.nb, B.h_fams[B.fam_ids[f]]);
 }
 
 // persistent workgroups of the stage kernel on this device
@@ -306,8 +322,8 @@ void launch_births_ag(hipStream_t s, const FamArgs& a, const long long* cell, co
 void launch_stage_mk(const StageLaunch& L, const FamArgs& a);
 void launch_stage_ig(const StageLaunch& L, const FamArgs& a);
 void launch_stage_ag(const StageLaunch& L, const FamArgs& a);
-void launch_dt_mk(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a);
-void launch_dt_ig(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a);
-void launch_dt_ag(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a);
+void launch_dt_mk(hipStream_t s, const int* wg, int nwg, int nb, const FamArgs& a);
+void launch_dt_ig(hipStream_t s, const int* wg, int nwg, int nb, const FamArgs& a);
+void launch_dt_ag(hipStream_t s, const int* wg, int nwg, int nb, const FamArgs& a);
 
 }  // namespace ice

@@ -6,7 +6,9 @@ void launch_stage_ag(const StageLaunch& L, const FamArgs& a) {
   if (L.riemann == 2) launch_sd_cp<2, 2>(L, a);
   else launch_sd_cp<2, 1>(L, a);
 }
-void launch_dt_ag(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<2>(s, blk, nb, a); }
+void launch_dt_ag(hipStream_t s, const int* wg, int nwg, int nb, const FamArgs& a) {
+  launch_dt_kernel<2>(s, wg, nwg, nb, a);
+}
 void launch_ghosts_ag(hipStream_t s, const int* grp, int ngrp, const FamArgs& a) {
   launch_ghost_kernels<2>(s, grp, ngrp, a);
 }

@@ -6,7 +6,9 @@ void launch_stage_ig(const StageLaunch& L, const FamArgs& a) {
   if (L.riemann == 2) launch_sd_cp<1, 2>(L, a);
   else launch_sd_cp<1, 1>(L, a);
 }
-void launch_dt_ig(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<1>(s, blk, nb, a); }
+void launch_dt_ig(hipStream_t s, const int* wg, int nwg, int nb, const FamArgs& a) {
+  launch_dt_kernel<1>(s, wg, nwg, nb, a);
+}
 void launch_ghosts_ig(hipStream_t s, const int* grp, int ngrp, const FamArgs& a) {
   launch_ghost_kernels<1>(s, grp, ngrp, a);
 }

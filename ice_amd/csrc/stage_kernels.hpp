@@ -122,19 +122,6 @@ ICE_DEV double dt_cell(const FamArgs& a, long long g) {
   return dt;
 }
 
-template <int M>
-__global__ void __launch_bounds__(kThreads) k_dt(FamArgs a, int b) {
-  const BlkDev B = a.blk[b];
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  double dtc = 1.79769313486231571e308;
-  if (t < B.cells) {
-    int i, j, k;
-    cell_of(B, t, i, j, k);
-    dtc = dt_cell<M>(a, gidx(B, i, j, k));
-  }
-  block_min_atomic(dtc, &a.st->dtbits);
-}
-
 // ------------------------------------------------------------------ ghosts
 enum { kCopy = 0, kMirror = 1, kInject = 2 };
 
@@ -807,6 +794,68 @@ __global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int d, const double
   for (int v = 0; v < NC; ++v) dst[(long long)v * a.N + g] = r[v];
 }
 
+// Direction d = 1 (j) or 2 (k) of the smoothing, marching: one thread per line and
+// slot (blockIdx.y), lanes along i, down the line.  Each residual is read once and
+// each first-sweep value T(j) formed once, by irs_line's expressions with its end
+// copies (R(0) = R(1), R(n+1) = R(n); T(0) = R(1), T(n+1) = R(n)), so the result is
+// irs_line's bit for bit.  Workgroups wl[b] .. wl[b+1]-1 take block b's lines.
+template <int NC>
+__global__ void __launch_bounds__(kThreads) k_irs_march(FamArgs a, int d, const double* __restrict__ src,
+                                                        double* __restrict__ dst, const int* __restrict__ wl, int nb) {
+  const int w = (int)blockIdx.x;
+  const int b = block_of_workgroup(wl, nb, w);
+  const BlkDev B = a.blk[b];
+  const int no = d == 1 ? B.nk : B.nj;
+  const long long t = (long long)(w - wl[b]) * blockDim.x + threadIdx.x;
+  if (t >= (long long)B.ni * no) return;
+  const int i = (int)(t % B.ni) + 1, o = (int)(t / B.ni) + 1;
+  const int n = d == 1 ? B.nj : B.nk;
+  const long long s = dir_stride(B, d);
+  long long g = d == 1 ? gidx(B, i, 1, o) : gidx(B, i, o, 1);
+  const int v = (int)blockIdx.y;
+  const double* __restrict__ R = src + (long long)v * a.N;
+  double* __restrict__ D = dst + (long long)v * a.N;
+  const double beta = a.irs_beta;
+  const double den = 1.0 + 2.0 * beta;
+  double r0 = R[g];                           // R(m)
+  double rp = n > 1 ? R[g + s] : r0;          // R(m+1)
+  double tm = r0;                             // T(m-1)
+  double tc = (r0 + beta * (r0 + rp)) / den;  // T(m)
+  int m = 1;
+  // interior: R(m+2) is a cell of the line
+#pragma unroll 4
+  for (; m + 2 <= n; ++m) {
+    const double rpp = R[g + 2 * s];
+    const double tp = (rp + beta * (r0 + rpp)) / den;
+    D[g] = (r0 + beta * (tm + tp)) / den;
+    tm = tc; tc = tp; r0 = rp; rp = rpp; g += s;
+  }
+  if (m + 1 == n) {  // m = n-1: R(n+1) = R(n)
+    const double tp = (rp + beta * (r0 + rp)) / den;
+    D[g] = (r0 + beta * (tm + tp)) / den;
+    tm = tc; r0 = rp; g += s; ++m;
+  }
+  // m = n: T(n+1) = R(n)
+  D[g] = (r0 + beta * (tm + r0)) / den;
+}
+
+// the dt pass of every block of the family in one launch (workgroups wg[b] ..
+// wg[b+1]-1 take block b's cells, one per thread)
+template <int M>
+__global__ void __launch_bounds__(kThreads) k_dt_blocks(FamArgs a, const int* __restrict__ wg, int nb) {
+  const int w = (int)blockIdx.x;
+  const int b = block_of_workgroup(wg, nb, w);
+  const BlkDev B = a.blk[b];
+  long long t = (long long)(w - wg[b]) * blockDim.x + threadIdx.x;
+  double dtc = 1.79769313486231571e308;
+  if (t < B.cells) {
+    int i, j, k;
+    cell_of(B, t, i, j, k);
+    dtc = dt_cell<M>(a, gidx(B, i, j, k));
+  }
+  block_min_atomic(dtc, &a.st->dtbits);
+}
+
 // the update of every block of the family from the smoothed residual res, in one
 // launch (workgroups wg[b] .. wg[b+1]-1 take block b, as in k_irs)
 template <int M>
@@ -922,12 +971,17 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
   if (!ok) atomicOr(&a.st->bad_recon, 1);
   // a cell whose source runs the Langmuir-Knudsen solve finishes in k_lk_finish,
   // where such cells fill whole waves instead of holding up the rest of theirs
-  bool deferred = false;
-  if constexpr (COUPLED) {
-    if (a.lkq_g) deferred = defer_cell<NC>(a, has_cell && lk_cell_needed<M, TAB>(a, g), g, b, r);
-  }
+  // (with the queue, only a cell past its capacity needs the seed here)
+  bool deferred = false, seeded = has_cell;
   float seed = 0.0f;
-  if constexpr (COUPLED) seed = stage_seed<M, TAB>(a, has_cell && !deferred, g, mlds);
+  if constexpr (COUPLED) {
+    if (a.lkq_g) {
+      const bool need = has_cell && lk_cell_needed<M, TAB>(a, g);
+      deferred = defer_cell<NC>(a, need, g, b, r);
+      seeded = need && !deferred;
+    }
+    seed = stage_seed<M, TAB>(a, seeded, g, mlds);
+  }
   if (!has_cell || deferred) return;
   finish_cell<M, TAB, COUPLED, WRITE_R>(a, Bref, ci, cj, ck, g, r, seed);
 }
@@ -1015,7 +1069,7 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_items(
   stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
 }
 
-// the sensor weights and the dt pass over the same queues, one cell per thread
+// the sensor weights over the same queues, one cell per thread
 template <int M>
 __global__ void __launch_bounds__(kThreads) k_beta_persist(const FamArgs* fams, WorkQueues Q) {
   __shared__ int slot;
@@ -1043,27 +1097,6 @@ __global__ void __launch_bounds__(kThreads) k_beta_persist(const FamArgs* fams, 
   }
 }
 
-template <int M>
-__global__ void __launch_bounds__(kThreads) k_dt_persist(const FamArgs* fams, WorkQueues Q) {
-  __shared__ int slot;
-  int q = blockIdx.x % Q.nq;
-  double dmin = 1.79769313486231571e308;
-  DevStatus* st = fams[0].st;
-  for (;;) {
-    int it = next_item(Q, q, &slot);
-    if (it < 0) break;
-    const WorkItem w = Q.items[it];
-    const FamArgs& a = fams[w.fam];
-    const BlkDev B = a.blk[w.block];
-    const Tile& T = w.T;
-    int t = threadIdx.x;
-    int ox = (w.tile % T.ntx) * T.tx + 1, oy = ((w.tile / T.ntx) % T.nty) * T.ty + 1,
-        oz = (w.tile / (T.ntx * T.nty)) * T.tz + 1;
-    int ci = ox + t % T.tx, cj = oy + (t / T.tx) % T.ty, ck = oz + t / (T.tx * T.ty);
-    if (t < T.tx * T.ty * T.tz && ci <= B.ni && cj <= B.nj && ck <= B.nk) dmin = dmin2(dmin, dt_cell<M>(a, gidx(B, ci, cj, ck)));
-  }
-  block_min_atomic(dmin, &st->dtbits);
-}
 
 namespace {  // one copy per translation unit
 __global__ void k_zero_tickets(unsigned long long* t, int n) {

@@ -7,7 +7,9 @@ void launch_stage_mk(const StageLaunch& L, const FamArgs& a) {
   else if (L.riemann == 1) launch_sd_cp<0, 1>(L, a);
   else launch_sd_cp<0, 2>(L, a);
 }
-void launch_dt_mk(hipStream_t s, const BlkDev* blk, int nb, const FamArgs& a) { launch_dt_kernel<0>(s, blk, nb, a); }
+void launch_dt_mk(hipStream_t s, const int* wg, int nwg, int nb, const FamArgs& a) {
+  launch_dt_kernel<0>(s, wg, nwg, nb, a);
+}
 void launch_ghosts_mk(hipStream_t s, const int* grp, int ngrp, const FamArgs& a) {
   launch_ghost_kernels<0>(s, grp, ngrp, a);
 }
