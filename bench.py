@@ -158,11 +158,24 @@ def traffic_of(workload: str, mode: str, cells_per_launch: float):
 
 def autotune(gpu, run, steps: int = 2):
     best = np.zeros(4, np.int32)
-    trial_ms = np.zeros(16)
+    trial_ms = np.zeros(24)
     if gpu.ice_gpu_autotune(run.ctx, steps, best, trial_ms) != 0:
         raise RuntimeError(run.error())
     return {"tile": best[:3].tolist(), "dispatch": int(best[3]),
             "trial_ms_per_step": [round(float(x), 4) for x in trial_ms if np.isfinite(x) and x > 0]}
+
+
+def device_used_gb():
+    """Device memory in use (hipMemGetInfo: total - free), GB; None if the runtime
+    is not found."""
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    free, total = C.c_size_t(), C.c_size_t()
+    if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        return None
+    return (total.value - free.value) / 1e9
 
 
 def timed_steps(run, steps: int) -> float:
@@ -252,7 +265,7 @@ def main():
                     help="keep the default stage tile (otherwise the library's autotuner picks it, untimed)")
     ap.add_argument("--tile", default=None,
                     help="stage tile tx,ty,tz (with --dispatch): set instead of autotuned, as for a profile")
-    ap.add_argument("--dispatch", type=int, default=None, help="batched dispatch with --tile: 0 queues, 1 items")
+    ap.add_argument("--dispatch", type=int, default=None, help="batched dispatch with --tile: 0 queues, 1 items, 2 items family by family")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB three times on the device (the PMC traffic calibration kernel) and exit")
     ap.add_argument("--cpu-steps", type=int, default=50,
@@ -305,7 +318,8 @@ def main():
     for f in case.fams:
         f.surrogate = 1 if seeded else 0
     run = cases.Runner(gpu, case, "ice_gpu_", workers=8)
-    progress("context set up")
+    used_gb = device_used_gb()
+    progress("context set up" + (f" ({used_gb:.1f} GB of device memory in use)" if used_gb else ""))
     surrogate = None
     if seeded:
         surrogate = {"retrain_every_steps": 200, "training": []}
@@ -412,7 +426,7 @@ def main():
                        "; learned BF16 MFMA seed of the interface solve in the loop, trained before the timed steps "
                        "and retrained every 200 steps (no retraining falls inside the timed steps)" if seeded else ""),
                    "mode": args.mode + ("+surrogate" if seeded else ""), "cells": case.cells,
-                   "families": case.cfg.ngroups, "rk_stages": case.cfg.nrk,
+                   "families": case.cfg.ngroups, "rk_stages": case.cfg.nrk, "device_gb_in_use": used_gb,
                    "event_ms_per_step": step_event_ms / args.steps, "iter_before": it0.value, "iter_after": it1.value,
                    "autotune": tuned, "compared": compared, "interface_solve": lk},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

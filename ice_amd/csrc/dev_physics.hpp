@@ -22,6 +22,39 @@ namespace ice {
 
 #define ICE_DEV __device__ __forceinline__
 
+// ------------------------------------------------------------ shared divisor
+// Quotients x / d of several numerators over one divisor, each bit for bit the
+// IEEE quotient that gfx950's division gives.  That division scales the divisor
+// (v_div_scale), refines its reciprocal (v_rcp and two Newton steps), forms and
+// corrects the quotient (v_div_fmas) and fixes the special cases (v_div_fixup).
+// Only the scaled divisor and its refined reciprocal are shared here; the divisor
+// is scaled for the numerator x only at extreme exponents, and such a numerator
+// gets the plain division.
+struct SharedDiv {
+  double d, d0, r;
+};
+ICE_DEV SharedDiv div_prepare(double d) {
+  bool f;
+  SharedDiv s;
+  s.d = d;
+  s.d0 = __builtin_amdgcn_div_scale(1.0, d, false, &f);
+  const double r0 = __builtin_amdgcn_rcp(s.d0);
+  const double e0 = fma(-s.d0, r0, 1.0);
+  const double r1 = fma(r0, e0, r0);
+  const double e1 = fma(-s.d0, r1, 1.0);
+  s.r = fma(r1, e1, r1);
+  return s;
+}
+ICE_DEV double div_shared(double x, const SharedDiv& s) {
+  bool f, vcc;
+  const double d0 = __builtin_amdgcn_div_scale(x, s.d, false, &f);
+  if (__builtin_expect(__double_as_longlong(d0) != __double_as_longlong(s.d0), 0)) return x / s.d;
+  const double n0 = __builtin_amdgcn_div_scale(x, s.d, true, &vcc);
+  const double q0 = n0 * s.r;
+  const double e = fma(-s.d0, q0, n0);
+  return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(e, s.r, q0, vcc), s.d, x);
+}
+
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kSigmaSB = 5.670374419e-8;
 constexpr double kRhoEmpty = 1e-6;
@@ -701,11 +734,12 @@ ICE_DEV void riemann_acc(const double* q1, const double* q4, const double* n, do
       }
     } else {
       double sq1 = sqrt(q1[0]), sq4 = sqrt(q4[0]);
-      double uR = (sq1 * q1[1] + sq4 * q4[1]) / (sq1 + sq4);
-      double vR = (sq1 * q1[2] + sq4 * q4[2]) / (sq1 + sq4);
-      double wR = (sq1 * q1[3] + sq4 * q4[3]) / (sq1 + sq4);
+      const SharedDiv bys = div_prepare(sq1 + sq4);
+      double uR = div_shared(sq1 * q1[1] + sq4 * q4[1], bys);
+      double vR = div_shared(sq1 * q1[2] + sq4 * q4[2], bys);
+      double wR = div_shared(sq1 * q1[3] + sq4 * q4[3], bys);
       double vnR = uR * n[0] + vR * n[1] + wR * n[2];
-      double aR = (sq1 * a1 + sq4 * a4) / (sq1 + sq4);
+      double aR = div_shared(sq1 * a1 + sq4 * a4, bys);
       double S1 = dmin2(dmin2(0.0, vn1 - a1), vnR - aR);
       double S4 = dmax2(dmax2(0.0, vn4 + a4), vnR + aR);
       double k0 = -S1, k1 = S1 * S4, k2 = S4;
@@ -722,11 +756,12 @@ ICE_DEV void riemann_acc(const double* q1, const double* q4, const double* n, do
           r[v] = MODE == 2 ? fv * area : (MODE == 1 ? r[v] - fv * area : r[v] + fv * area);
         }
       } else {
+        const SharedDiv byw = div_prepare(S4 - S1);
 #pragma unroll
         for (int v = 0; v < NC; ++v) {
-          double fv = (S4 * flux_v<M>(v, q1, n, s1, m) - S1 * flux_v<M>(v, q4, n, s4, m) +
-                       S1 * S4 * (cons_v<M>(v, q4, m) - cons_v<M>(v, q1, m))) /
-                      (S4 - S1);
+          double fv = div_shared(S4 * flux_v<M>(v, q1, n, s1, m) - S1 * flux_v<M>(v, q4, n, s4, m) +
+                                     S1 * S4 * (cons_v<M>(v, q4, m) - cons_v<M>(v, q1, m)),
+                                 byw);
           r[v] = MODE == 2 ? fv * area : (MODE == 1 ? r[v] - fv * area : r[v] + fv * area);
         }
       }
@@ -758,18 +793,20 @@ ICE_DEV bool reconstruct_mem(int lim, const double* __restrict__ p, long long st
   }
   bool ok_l = first_l, ok_r = first_r;
   double limval = beta;
+  // the slopes of every slot divide by the same three spacings
+  const SharedDiv by0 = div_prepare(dl0), by1 = div_prepare(dl1), by2 = div_prepare(dl2);
   while (!ok_l || !ok_r) {
 #pragma unroll
     for (int v = 0; v < NC; ++v) {
       const double* pv = p + v * N;
       double prev = pv[0], local = pv[stride], next = pv[2 * stride], next2 = pv[3 * stride];
-      double s1 = (next - local) / dl1;
+      double s1 = div_shared(next - local, by1);
       if (!first_l) {
-        double s0 = (local - prev) / dl0;
+        double s0 = div_shared(local - prev, by0);
         ql[v] = local + limiter(lim, s1, s0, cxl, cyl) * limval * dll;
       }
       if (!first_r) {
-        double s2 = (next2 - next) / dl2;
+        double s2 = div_shared(next2 - next, by2);
         qr[v] = next - limiter(lim, s2, s1, cxr, cyr) * limval * dlr;
       }
     }

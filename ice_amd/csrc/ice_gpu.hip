@@ -1238,7 +1238,8 @@ void snapshot_free(Snapshot& S) {
 }  // namespace
 
 int ice_gpu_set_dispatch(ice_gpu_ctx* c, int dispatch) {
-  if (dispatch != 0 && dispatch != 1) return fail(c, "[ICE::set_dispatch] dispatch is 0 (queues) or 1 (items)");
+  if (dispatch < 0 || dispatch > 2)
+    return fail(c, "[ICE::set_dispatch] dispatch is 0 (queues), 1 (items) or 2 (items family by family)");
   c->dispatch = dispatch;
   batched_destroy(c);  // its captured graphs hold the old launches
   return 0;
@@ -1260,10 +1261,10 @@ int ice_gpu_autotune(ice_gpu_ctx* c, int steps, int32_t* best, double* ms_out) {
   double best_ms = std::numeric_limits<double>::infinity();
   int bi = -1, bd = c->dispatch;
   std::string err;
-  const int ndisp = c->cfg.mode == ICE_MODE_BATCHED ? 2 : 1;
+  const int ndisp = c->cfg.mode == ICE_MODE_BATCHED ? 3 : 1;
   for (int k = 0; k < ncand * ndisp; ++k) {
     double ms = std::numeric_limits<double>::infinity();
-    if (ndisp == 2) c->dispatch = k / ncand;
+    if (ndisp > 1) c->dispatch = k / ncand;
     if (ice_gpu_set_tiles(c, cand[k % ncand][0], cand[k % ncand][1], cand[k % ncand][2]) == 0) {
       ice_status_t st;
       // one untimed step (plan, graph capture), then the timed ones
@@ -1671,22 +1672,24 @@ int build_plan(ice_gpu_ctx* c) {
   }
   for (auto& g : P->groups) {
     // items block by block, tile by tile, the group's families side by side so a
-    // tile's geometry and gas are read from L2 by all of them
+    // tile's geometry and gas are read from L2 by all of them; with dispatch 2 each
+    // block's tiles family by family, so that a tile row's cells of one family (not
+    // of all) share an XCD's L2
     std::vector<WorkItem> items;
     for (size_t b = 0; b < c->blk.size(); ++b) {
       Tile T = stage_tile(c, c->blk[b]);
       if (T.tx == 0) return fail(c, "[ICE::batched] the batched mode needs a tiled stage kernel");
-      int nt = T.ntx * T.nty * T.ntz;
-      for (int t = 0; t < nt; ++t)
-        for (int p : g.fams) {
-          WorkItem w;
-          w.fam = p;
-          w.block = (int)b;
-          w.tile = t;
-          w.pad = 0;
-          w.T = T;
-          items.push_back(w);
-        }
+      const int nt = T.ntx * T.nty * T.ntz, nf = (int)g.fams.size();
+      for (int k = 0; k < nt * nf; ++k) {
+        const bool fam_major = c->dispatch == 2;
+        WorkItem w;
+        w.fam = g.fams[fam_major ? k / nt : k % nf];
+        w.block = (int)b;
+        w.tile = fam_major ? k % nt : k / nf;
+        w.pad = 0;
+        w.T = T;
+        items.push_back(w);
+      }
     }
     g.nitems = (int)items.size();
     g.nq = 8;

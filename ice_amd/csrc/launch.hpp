@@ -186,7 +186,7 @@ struct BatchLaunch {
   const StageLaunch* per_fam;  // record lists of each family of the group
   WorkQueues Q;
   int grid;                    // persistent workgroups
-  int dispatch;                // 0: persistent workgroups over the queues, 1: one workgroup per item
+  int dispatch;                // 0: persistent workgroups over the queues, 1, 2: one workgroup per item
   int nitems;
   int irs, ndir, nb;           // implicit residual smoothing: the stage writes R, then the sweeps
   const int* irs_wg;           // first workgroup of each block in the all-blocks cell launches
@@ -246,9 +246,9 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
   if (B.irs) {
     // residual into R, smoothed per block and direction, then the update
     const dim3 items((unsigned)B.nitems), grid(B.grid);
-    if (B.dispatch == 1 && B.limiter == 3)
+    if (B.dispatch != 0 && B.limiter == 3)
       hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, true, 3>), items, blk, 0, B.stream, B.d_fams, B.Q.items);
-    else if (B.dispatch == 1)
+    else if (B.dispatch != 0)
       hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, true, -1>), items, blk, 0, B.stream, B.d_fams, B.Q.items);
     else if (B.limiter == 3)
       hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, true, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q);
@@ -264,7 +264,7 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
     }
     return;
   }
-  if (B.dispatch == 1) {
+  if (B.dispatch != 0) {
     dim3 grid((unsigned)B.nitems);
     if (B.limiter == 1) hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, false, 1>), grid, blk, 0, B.stream, B.d_fams, B.Q.items);
     else if (B.limiter == 3) hipLaunchKernelGGL((k_stage_items<M, S, SD, CP, false, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q.items);

@@ -750,27 +750,30 @@ ICE_DEV int block_of_workgroup(const int* __restrict__ wg, int nb, int w) {
   return lo;
 }
 
-// the value direction d leaves at cell g (index m of n along d), for every slot
+// The value direction d leaves at cell g (index m of n along d), for every slot:
+// the first sweep's values at m-1 and m+1 (ghost copies at the line's ends, where
+// the value is the cell's own), then the second sweep.  The five cells g-2s ..
+// g+2s are all inside the block's padded box (two ghost layers), so their loads are
+// issued unconditionally and the ends select.
 template <int NC>
 ICE_DEV void irs_line(const FamArgs& a, const double* __restrict__ src, long long g, long long s, int m, int n,
                       double* out) {
   const double beta = a.irs_beta;
   const double den = 1.0 + 2.0 * beta;
+  double x[NC][5];
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
-    const double* R = src + (long long)v * a.N;
-    const double r0 = R[g];
-    double tm = r0, tp = r0;  // the first sweep's values at m-1 and m+1, or the ghost copies
-    if (m > 1) {
-      const double rm = R[g - s];
-      const double rmm = m - 1 > 1 ? R[g - 2 * s] : rm;
-      tm = (rm + beta * (rmm + r0)) / den;
-    }
-    if (m < n) {
-      const double rp = R[g + s];
-      const double rpp = m + 1 < n ? R[g + 2 * s] : rp;
-      tp = (rp + beta * (r0 + rpp)) / den;
-    }
+    const double* R = src + (long long)v * a.N + g;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) x[v][q] = R[(q - 2) * s];
+  }
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    const double r0 = x[v][2], rm = x[v][1], rp = x[v][3];
+    const double rmm = m - 1 > 1 ? x[v][0] : rm;
+    const double rpp = m + 1 < n ? x[v][4] : rp;
+    const double tm = m > 1 ? (rm + beta * (rmm + r0)) / den : r0;
+    const double tp = m < n ? (rp + beta * (r0 + rpp)) / den : r0;
     out[v] = (r0 + beta * (tm + tp)) / den;
   }
 }
@@ -822,8 +825,20 @@ __global__ void __launch_bounds__(kThreads) k_irs_march(FamArgs a, int d, const 
   double tm = r0;                             // T(m-1)
   double tc = (r0 + beta * (r0 + rp)) / den;  // T(m)
   int m = 1;
-  // interior: R(m+2) is a cell of the line
-#pragma unroll 4
+  // interior, eight cells at a time with their loads issued together: R(m+2) ..
+  // R(m+9) are cells of the line
+  for (; m + 9 <= n; m += 8) {
+    double x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = R[g + (2 + q) * s];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double tp = (rp + beta * (r0 + x[q])) / den;
+      D[g] = (r0 + beta * (tm + tp)) / den;
+      tm = tc; tc = tp; r0 = rp; rp = x[q]; g += s;
+    }
+  }
+  // the rest of the interior: R(m+2) is a cell of the line
   for (; m + 2 <= n; ++m) {
     const double rpp = R[g + 2 * s];
     const double tp = (rp + beta * (r0 + rpp)) / den;
@@ -893,7 +908,7 @@ constexpr int kMaxTileFaces = kThreads;
 // order), then the cell's source, residual and update as in k_stage.
 // One tile of the stage: the body of k_stage_tiled, also run by the persistent
 // kernel for the tiles it takes from its work queue.  fl is the workgroup's
-// LDS buffer of face fluxes.
+// LDS buffer of face fluxes, two halves of NC * kMaxTileFaces (alternate passes).
 // TAB = false: the launch serves only families without tabulated properties
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM, bool TAB = false>
 ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl, uint16_t* mlds) {
@@ -921,9 +936,15 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
   bool ok = true;
   const int nd[3] = {B.ni, B.nj, B.nk};
   const int td[3] = {T.tx, T.ty, T.tz};
+  // the faces of successive passes go to alternate halves of fl: a pass's faces
+  // are written after the barrier of the previous pass, which every wave reaches
+  // only when it has summed the faces of the pass before, so one barrier a pass
+  int pass = 0;
   for (int d = 0; d < 3; ++d) {
     int n = nd[d];
     if (n < 2) continue;
+    double* fp = fl + (pass & 1) * (NC * kMaxTileFaces);
+    ++pass;
     // faces of the tile along d: lower-cell offset -1 .. t_d-1 along d
     int fx = T.tx + (d == 0), fy = T.ty + (d == 1), fz = T.tz + (d == 2);
     int nf = fx * fy * fz;
@@ -938,7 +959,7 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
         double q[NC];
         ok = face_acc<M, S, SD, 2, LIM, TAB>(a, gidx(B, li, lj, lk), stride, d, m, n, q) && ok;
 #pragma unroll
-        for (int v = 0; v < NC; ++v) fl[v * kMaxTileFaces + f] = q[v];
+        for (int v = 0; v < NC; ++v) fp[v * kMaxTileFaces + f] = q[v];
       }
     }
     __syncthreads();
@@ -951,22 +972,21 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
       if (m % 2 == 1) {
         if (m < n) {
 #pragma unroll
-          for (int v = 0; v < NC; ++v) r[v] = r[v] - fl[v * kMaxTileFaces + fup];
+          for (int v = 0; v < NC; ++v) r[v] = r[v] - fp[v * kMaxTileFaces + fup];
         }
         if (m > 1) {
 #pragma unroll
-          for (int v = 0; v < NC; ++v) r[v] = r[v] + fl[v * kMaxTileFaces + flo];
+          for (int v = 0; v < NC; ++v) r[v] = r[v] + fp[v * kMaxTileFaces + flo];
         }
       } else {
 #pragma unroll
-        for (int v = 0; v < NC; ++v) r[v] = r[v] + fl[v * kMaxTileFaces + flo];
+        for (int v = 0; v < NC; ++v) r[v] = r[v] + fp[v * kMaxTileFaces + flo];
         if (m < n) {
 #pragma unroll
-          for (int v = 0; v < NC; ++v) r[v] = r[v] - fl[v * kMaxTileFaces + fup];
+          for (int v = 0; v < NC; ++v) r[v] = r[v] - fp[v * kMaxTileFaces + fup];
         }
       }
     }
-    __syncthreads();
   }
   if (!ok) atomicOr(&a.st->bad_recon, 1);
   // a cell whose source runs the Langmuir-Knudsen solve finishes in k_lk_finish,
@@ -1001,7 +1021,7 @@ ICE_DEV int xcd_tile(int w, int n) {
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_tiled(FamArgs a, int b, Tile T) {
   constexpr int NC = Closure<M>::NC;
-  __shared__ double fl[NC * kMaxTileFaces];
+  __shared__ double fl[2 * NC * kMaxTileFaces];
   __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
   stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(a, b, T, xcd_tile(blockIdx.x, gridDim.x), fl, mlds);
 }
@@ -1040,9 +1060,10 @@ ICE_DEV int next_item(const WorkQueues& Q, int q, int* shared_slot) {
 }
 
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
-__global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_persist(const FamArgs* fams, WorkQueues Q) {
+__global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_persist(const FamArgs* __restrict__ fams,
+                                                                                   WorkQueues Q) {
   constexpr int NC = Closure<M>::NC;
-  __shared__ double fl[NC * kMaxTileFaces];
+  __shared__ double fl[2 * NC * kMaxTileFaces];
   __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
   __shared__ int slot;
   int q = blockIdx.x % Q.nq;
@@ -1059,10 +1080,10 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_persis
 // contiguous runs (an item's tile neighbours and its sibling families share
 // that XCD's L2).  Same tiles, same bits as k_stage_persist.
 template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
-__global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_items(const FamArgs* fams,
-                                                                                 const WorkItem* items) {
+__global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_items(const FamArgs* __restrict__ fams,
+                                                                                 const WorkItem* __restrict__ items) {
   constexpr int NC = Closure<M>::NC;
-  __shared__ double fl[NC * kMaxTileFaces];
+  __shared__ double fl[2 * NC * kMaxTileFaces];
   __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
   const int it = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x));
   const WorkItem w = items[it];

@@ -142,7 +142,7 @@ module ice_gpu_m
       type(c_ptr), value :: ctx
       integer(c_int), value :: steps
       integer(c_int32_t) :: best(4)
-      real(c_double) :: ms(16)
+      real(c_double) :: ms(24)
     end function
     integer(c_int) function ice_gpu_set_properties(ctx, p, t_min, n, cols, flags) bind(C, name='ice_gpu_set_properties')
       import :: c_int, c_int32_t, c_double, c_ptr

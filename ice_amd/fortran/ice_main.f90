@@ -28,7 +28,7 @@ program ice_driver
   integer(c_int32_t), allocatable :: reci(:,:), donor(:,:), dims(:)
   real(R8), allocatable :: recd(:,:), weight(:), fam(:), prop_cols(:,:)
   integer :: t_min, prop_flags
-  real(R8) :: rho_mat, cs, total(5), step_ms, kernel_ms, tile_ms(16), res_ref
+  real(R8) :: rho_mat, cs, total(5), step_ms, kernel_ms, tile_ms(24), res_ref
   integer(c_int32_t) :: tile(4), stopped
   integer(c_int64_t) :: launches
   type(c_ptr) :: ctx

@@ -185,13 +185,15 @@ int  ice_gpu_timing(ice_gpu_ctx* ctx, double* step_ms, double* kernel_ms, int64_
  * tx = 0 selects the untiled one-thread-per-cell kernel. */
 int  ice_gpu_set_tiles(ice_gpu_ctx* ctx, int tx, int ty, int tz);
 /* Dispatch of the batched stage kernel: 0 persistent workgroups over per-XCD tile
- * queues, 1 one workgroup per tile (same tiles, same bits). */
+ * queues, 1 one workgroup per tile with the families of a tile side by side, 2 one
+ * workgroup per tile with each block's tiles family by family (same tiles, same
+ * bits). */
 int  ice_gpu_set_dispatch(ice_gpu_ctx* ctx, int dispatch);
 /* The autotuner: times `steps` steps of the run (plain or batched, as configured)
  * with each candidate stage tile for its blocks' dimensionality (and, batched, each
  * dispatch), restores the state after every trial, and keeps the fastest.
  * best[4] receives the tile and the dispatch, ms[k] the milliseconds per step of
- * trial k (at most 16; infinity when it did not fit).  Needs a device copy of the
+ * trial k (at most 24; infinity when it did not fit).  Needs a device copy of the
  * state (three buffers, tau and dt per family). */
 int  ice_gpu_autotune(ice_gpu_ctx* ctx, int steps, int32_t* best, double* ms);
 /* 1: bracket every launch of the dominant kernel with HIP events (bench only). */

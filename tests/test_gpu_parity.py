@@ -179,7 +179,7 @@ def test_autotune_keeps_state_and_bits(gpu, mode):
     ra.step(1)
     rb.step(1)
     best = np.zeros(4, np.int32)
-    ms = np.zeros(16)
+    ms = np.zeros(24)
     assert gpu.ice_gpu_autotune(ra.ctx, 2, best, ms) == 0, ra.error()
     print("tile", best.tolist(), "ms/step", ms.tolist())
     assert best[0] > 0 and np.isfinite(ms).any()
@@ -192,17 +192,21 @@ def test_autotune_keeps_state_and_bits(gpu, mode):
 
 
 def test_item_dispatch_equals_queues(gpu):
-    """The batched stage as one workgroup per item gives the queues' bits."""
+    """The batched stage as one workgroup per item, in either item order, gives the
+    queues' bits."""
     case = cases.case_multiblock(nblocks=6, emin=8, emax=20, steps=4, mode=1)
-    print("case multiblock 6 blocks dispatch", flush=True)
     ra = cases.Runner(gpu, case, "ice_gpu_")
-    rb = cases.Runner(gpu, case, "ice_gpu_")
-    assert gpu.ice_gpu_set_dispatch(rb.ctx, 1) == 0
     ra.step(4)
-    rb.step(4)
-    for b in range(1, len(case.blocks) + 1):
-        for p in range(1, case.cfg.ngroups + 1):
-            assert np.array_equal(ra.state(b, p), rb.state(b, p))
+    for dispatch in (1, 2):
+        print("case multiblock 6 blocks dispatch", dispatch, flush=True)
+        rb = cases.Runner(gpu, case, "ice_gpu_")
+        assert gpu.ice_gpu_set_dispatch(rb.ctx, dispatch) == 0
+        rb.step(4)
+        for b in range(1, len(case.blocks) + 1):
+            for p in range(1, case.cfg.ngroups + 1):
+                assert np.array_equal(ra.state(b, p), rb.state(b, p))
+        rb.close()
+    assert gpu.ice_gpu_set_dispatch(ra.ctx, 3) != 0
 
 
 def test_combined_batched_irs_evap_chimera(gpu, oracle):
