@@ -494,11 +494,11 @@ Tile stage_tile(ice_gpu_ctx* c, const BlkDev& B) {
   if (c->tile[0] > 0) {
     tx = c->tile[0]; ty = c->tile[1]; tz = c->tile[2];
   } else if (B.nj == 1 && B.nk == 1) {
-    tx = 255; ty = 1; tz = 1;
+    tx = 254; ty = 1; tz = 1;
   } else if (B.nk == 1) {
-    tx = 15; ty = 16; tz = 1;
+    tx = 15; ty = 15; tz = 1;
   } else {
-    tx = 7; ty = 6; tz = 5;
+    tx = 6; ty = 6; tz = 5;
   }
   tx = std::min(tx, B.ni); ty = std::min(ty, B.nj); tz = std::min(tz, B.nk);
   T.tx = tx; T.ty = ty; T.tz = tz;
@@ -528,7 +528,8 @@ void dispatch_stage(ice_gpu_ctx* c, FamArgs& a) {
   L.tiles = tiles.data();
   L.nb = (int)c->blk.size();
   L.grp = F.grp; L.ngrp = F.ngrp; L.own = F.own; L.nown = F.nown; L.nbcell = F.nbcell;
-  L.riemann = c->fams[a.fam].riemann; L.sd = c->cfg.shock_detector; L.coupled = c->cfg.coupled;
+  L.riemann = c->fams[a.fam].riemann; L.sd = c->cfg.shock_detector;
+  L.coupled = c->cfg.coupled ? (c->fams[a.fam].evap_select != 0 ? 2 : 1) : 0;
   L.irs = c->cfg.irs; L.limiter = c->cfg.limiter; L.ndir = c->cfg.ndir;
   L.irs_wg = c->d_irs_wg; L.irs_nwg = c->irs_nwg;
   for (int d = 0; d < 2; ++d) { L.irs_wl[d] = c->d_irs_wl[d]; L.irs_nwl[d] = c->irs_nwl[d]; }
@@ -701,12 +702,13 @@ int ice_gpu_create(int device, const ice_config_t* cfg, const ice_family_t* fams
       if (dalloc(c, &F.R, (size_t)F.nc * N) || dalloc(c, &F.T, (size_t)F.nc * N)) return 1;
     }
     if (cfg->coupled && fams[p].evap_select != 0 && fams[p].intf_select == 1) {
-      // room for every cell of a grid up to 16 M cells, for 45 % of a larger one (a cell
-      // past the capacity finishes on its tile)
+      // room for every interior cell: the tile kernels leave each cell that needs the
+      // interface solve to k_lk_finish (108 B per cell for AG, 60 for MK)
       long long cells = 0;
       for (const BlkDev& B : c->blk) cells += B.cells;
-      long long cap = cells <= (16ll << 20) ? cells : std::max(16ll << 20, cells * 45 / 100);
-      F.lkq_cap = (int)std::min<long long>(cap, (1ll << 31) - 1);
+      if (cells > (1ll << 31) - 1)
+        return fail(c, "[ICE::create] %lld cells: the interface-solve queue holds at most 2^31-1", cells);
+      F.lkq_cap = (int)cells;
       if (dalloc(c, &F.lkq_n, 1) || dalloc(c, &F.lkq_g, F.lkq_cap) || dalloc(c, &F.lkq_b, F.lkq_cap) ||
           dalloc(c, &F.lkq_r, (size_t)F.nc * F.lkq_cap))
         return 1;
@@ -1016,6 +1018,7 @@ int ice_gpu_step(ice_gpu_ctx* c, int nsteps, ice_status_t* st) {
     st->dtglobal = s.dtglobal;
     st->time = s.time;
   }
+  if (s.lk_overflow) return fail(c, " [ERROR] [ICE::compute_source] the queue of deferred interface solves overflowed");
   if (s.bad_recon) return fail(c, " [ERROR] [ICE::compute_flux] unphysical state at first order on a face");
   if (s.bad_state)
     return fail(c,
@@ -1134,10 +1137,12 @@ int ice_gpu_set_tiles(ice_gpu_ctx* c, int tx, int ty, int tz) {
     int n[3] = {B.ni, B.nj, B.nk}, t[3] = {std::min(tx, B.ni), std::min(ty, B.nj), std::min(tz, B.nk)};
     bool bad = ty < 1 || tz < 1 || t[0] * t[1] * t[2] > kThreads;
     for (int d = 0; d < 3; ++d)
-      if (n[d] > 1) bad = bad || t[0] * t[1] * t[2] / t[d] * (t[d] + 1) > kMaxTileFaces;
+      if (n[d] > 1)
+        bad = bad || t[0] * t[1] * t[2] / t[d] * (t[d] + 1) > kMaxTileFaces ||
+              t[0] * t[1] * t[2] / t[d] * (t[d] + 2) > kMaxExtCells;
     if (bad)
-      return fail(c, "[ICE::set_tiles] a %d x %d x %d tile has more cells or faces than the %d threads of a workgroup",
-                  tx, ty, tz, kThreads);
+      return fail(c, "[ICE::set_tiles] a %d x %d x %d tile has more cells or faces than the %d threads of a workgroup, "
+                  "or more than %d cells widened by one along a direction", tx, ty, tz, kThreads, kMaxExtCells);
 
   }
   c->tile[0] = tx; c->tile[1] = ty; c->tile[2] = tz;
@@ -1150,9 +1155,11 @@ int ice_gpu_set_tiles(ice_gpu_ctx* c, int tx, int ty, int tz) {
 // directions; each must keep its cells and the faces of each direction within
 // the 256 threads of a workgroup (checked by ice_gpu_set_tiles).
 namespace {
-const int kTiles3[][3] = {{7, 6, 5}, {6, 6, 6}, {7, 7, 4}, {10, 5, 4}, {12, 4, 4}, {15, 4, 3}, {14, 6, 2}, {8, 4, 7}};
-const int kTiles2[][3] = {{15, 16, 1}, {16, 15, 1}, {25, 9, 1}, {12, 20, 1}, {20, 12, 1}, {10, 24, 1}};
-const int kTiles1[][3] = {{255, 1, 1}, {128, 1, 1}, {192, 1, 1}};
+// (most of them widened by one cell along any direction still fit the 256 threads,
+// so each direction's limited slopes take one pass of the workgroup)
+const int kTiles3[][3] = {{6, 6, 5}, {7, 5, 5}, {5, 6, 6}, {6, 5, 6}, {7, 6, 4}, {8, 5, 4}, {7, 7, 4}, {7, 6, 5}};
+const int kTiles2[][3] = {{15, 15, 1}, {14, 16, 1}, {16, 14, 1}, {12, 18, 1}, {18, 12, 1}, {15, 16, 1}};
+const int kTiles1[][3] = {{254, 1, 1}, {128, 1, 1}, {192, 1, 1}};
 
 // a device copy of everything a step changes, to run trial steps and put it back
 struct Snapshot {
@@ -1699,8 +1706,14 @@ int build_plan(ice_gpu_ctx* c) {
       return 1;
     HIPCHK(hipMemcpy(g.d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(g.d_qstart, qs.data(), qs.size() * sizeof(int), hipMemcpyHostToDevice));
-    int wgs = g.closure == ICE_MK ? persistent_wgs_mk(P->cus) : (g.closure == ICE_IG ? persistent_wgs_ig(P->cus)
-                                                                                       : persistent_wgs_ag(P->cus));
+    int cp = 0;
+    if (c->cfg.coupled) {
+      cp = 1;
+      for (int p : g.fams)
+        if (c->fams[p].evap_select != 0) cp = 2;
+    }
+    int wgs = g.closure == ICE_MK ? persistent_wgs_mk(P->cus, cp)
+                                  : (g.closure == ICE_IG ? persistent_wgs_ig(P->cus, cp) : persistent_wgs_ag(P->cus, cp));
     g.grid = std::max(g.nq, (wgs / g.nq) * g.nq);
   }
   return 0;
@@ -1783,7 +1796,12 @@ BatchLaunch group_launch(ice_gpu_ctx* c, const BatchGroup& g, const std::vector<
   B.blk = c->blk.data();
   B.riemann = g.riemann;
   B.sd = c->cfg.shock_detector;
-  B.coupled = c->cfg.coupled;
+  B.coupled = 0;
+  if (c->cfg.coupled) {
+    B.coupled = 1;
+    for (int p : g.fams)
+      if (c->fams[p].evap_select != 0) B.coupled = 2;
+  }
   B.limiter = c->cfg.limiter;
   B.lk_grid = c->lk_grid;
   return B;

@@ -38,6 +38,7 @@ struct DevStatus {
   int bad_state, bad_recon;
   int bad_b, bad_i, bad_j, bad_k, bad_p;
   int iter;                   // grid%iter of the step being computed
+  int lk_overflow;            // a deferred interface solve found its family's queue full
 };
 
 // Everything a kernel of one family reads or writes
@@ -179,7 +180,11 @@ template <bool TAB = true> ICE_DEV void cell_ep(const FamArgs& a, long long g, d
 }
 
 // seed > 0: the interface solve starts from seed x (equilibrium rate), see evaporation_d
-template <int M, bool TAB = true>
+// EVAP = false: a launch whose families do not evaporate (the evaporation code is
+// not compiled into it, which frees its registers).  LK = false: a launch whose
+// cells needing the Langmuir-Knudsen interface solve are all deferred to
+// k_lk_finish (the solve is not compiled into it).
+template <int M, bool TAB = true, bool EVAP = true, bool LK = true>
 ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double* src, double& tau, float seed = 0.0f) {
   constexpr int NC = Closure<M>::NC;
   const long long N = a.N;
@@ -205,10 +210,10 @@ ICE_DEV void cell_source(const FamArgs& a, long long g, const double* q, double*
   double Qconv = 2.0 * Nu * gk * kPi * Rp * (gp[4] - Ts);
   double mdot = 0.0, Qevap = 0.0;
   bool ovr = false;
-  if (mt.evap != 0) {
+  if (EVAP && mt.evap != 0) {
     double ep[kNepCell];
     cell_ep<TAB>(a, g, Ts, ep);
-    evaporation_d(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, mt.evap, mt.intf, ep, mdot, Qevap, ovr, seed,
+    evaporation_d(gp[0], gp[4], gam, R, mu, gk, Ts, 2.0 * Rp, Re, mt.evap, LK ? mt.intf : 0, ep, mdot, Qevap, ovr, seed,
                   LkStats{a.lk_stats});
   }
   if (ovr) Qconv = Qevap;

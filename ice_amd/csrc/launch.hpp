@@ -19,7 +19,8 @@ struct StageLaunch {
   const int* own;           // owner records of its connection faces
   int nown;
   int nbcell;
-  int riemann, sd, coupled, irs, limiter, ndir;
+  int riemann, sd, irs, limiter, ndir;
+  int coupled;              // 0 uncoupled, 1 coupled, 2 coupled with an evaporating family
   const int* irs_wg;        // first workgroup of each block in the all-blocks cell launches (IRS along i,
   int irs_nwg;              // the update, the dt pass), and the workgroups of such a launch
   const int* irs_wl[2];     // the same for the lines along j and k of the marching smoothing
@@ -62,7 +63,7 @@ const double* launch_irs(const FamArgs& a, int ndir, const int* wg, int nwg, con
   return src;
 }
 
-template <int M, int S, bool SD, bool CP>
+template <int M, int S, bool SD, int CP>
 void launch_tiled(const StageLaunch& L, FamArgs& a, int b) {
   const Tile T = L.tiles[b];
   dim3 grid((unsigned)(T.ntx * T.nty * T.ntz)), blk(kThreads);
@@ -78,7 +79,7 @@ void launch_tiled(const StageLaunch& L, FamArgs& a, int b) {
   }
 }
 
-template <int M, int S, bool SD, bool CP>
+template <int M, int S, bool SD, int CP>
 void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
   constexpr int NC = Closure<M>::NC;
   const int nb = L.nb;
@@ -97,7 +98,7 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
     hipLaunchKernelGGL((k_conn<M, S, SD>), dim3(grid_cells(L.nown)), dim3(kThreads), 0, L.stream, a);
   }
   if (L.nbcell) hipLaunchKernelGGL((k_bcell<M, S, SD>), dim3(grid_cells(L.nbcell)), dim3(kThreads), 0, L.stream, a);
-  if (CP) clear_lk_queue(a, L.stream);
+  if (CP == 2) clear_lk_queue(a, L.stream);
   for (int b = 0; b < nb; ++b) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool timed = L.timing && !L.irs;
@@ -122,7 +123,7 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
       L.kev->push_back(e1);
     }
   }
-  if (CP && a.lkq_g) {
+  if (CP == 2 && a.lkq_g) {
     // bracketed as one more launch of the stage's kernels when timing
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool timed = L.timing && !L.irs;
@@ -145,11 +146,13 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
   }
 }
 
+// One closure, Riemann solver and sensor setting's stage kernels: instantiated in
+// stage_unit.hip, one object per combination
+template <int M, int S, bool SD> void launch_stage_unit(const StageLaunch& L, const FamArgs& a);
+
 template <int M, int S> void launch_sd_cp(const StageLaunch& L, const FamArgs& a) {
-  if (L.sd && L.coupled) launch_stage_kernels<M, S, true, true>(L, a);
-  else if (L.sd) launch_stage_kernels<M, S, true, false>(L, a);
-  else if (L.coupled) launch_stage_kernels<M, S, false, true>(L, a);
-  else launch_stage_kernels<M, S, false, false>(L, a);
+  if (L.sd) launch_stage_unit<M, S, true>(L, a);
+  else launch_stage_unit<M, S, false>(L, a);
 }
 
 // the two ghost layers of a family's current state (compute_ghost, fill_second_ghost)
@@ -194,14 +197,15 @@ struct BatchLaunch {
   const int* irs_wl[2];        // the same for the lines along j and k
   int irs_nwl[2];
   const BlkDev* blk;           // host copies of the blocks
-  int riemann, sd, coupled, limiter;
+  int riemann, sd, limiter;
+  int coupled;                 // 0 uncoupled, 1 coupled, 2 coupled with an evaporating family in the group
   int lk_grid;                 // workgroups of k_lk_finish
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // recorded around the persistent stage kernel when set
 };
 
 // the deferred interface solves of the group's families, after the stage kernel
-template <int M, bool CP> void finish_lk_group(const BatchLaunch& B, bool write_r) {
-  if (!CP) return;
+template <int M, int CP> void finish_lk_group(const BatchLaunch& B, bool write_r) {
+  if (CP != 2) return;
   for (int f = 0; f < B.nfam; ++f) {
     const FamArgs& a = B.h_fams[B.fam_ids[f]];
     if (write_r) launch_lk_finish<M, true>(a, B.lk_grid, B.stream);
@@ -213,7 +217,7 @@ inline void reset_tickets(const BatchLaunch& B) {
   hipLaunchKernelGGL(k_zero_tickets, dim3(1), dim3(64), 0, B.stream, B.Q.ticket, B.Q.nq);
 }
 
-template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchLaunch& B) {
+template <int M, int S, bool SD, int CP> void launch_batched_stage(const BatchLaunch& B) {
   for (int f = 0; f < B.nfam; ++f) {
     const StageLaunch& L = B.per_fam[f];
     FamArgs a = B.h_fams[B.fam_ids[f]];
@@ -239,7 +243,7 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
     if (L.nbcell) hipLaunchKernelGGL((k_bcell<M, S, SD>), dim3(grid_cells(L.nbcell)), dim3(kThreads), 0, B.stream, a);
   }
   reset_tickets(B);
-  if (CP)
+  if (CP == 2)
     for (int f = 0; f < B.nfam; ++f) clear_lk_queue(B.h_fams[B.fam_ids[f]], B.stream);
   if (B.ev0) (void)hipEventRecord(B.ev0, B.stream);
   dim3 blk(kThreads);
@@ -279,11 +283,12 @@ template <int M, int S, bool SD, bool CP> void launch_batched_stage(const BatchL
   if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
 }
 
+template <int M, int S, bool SD> void launch_batched_unit(const BatchLaunch& B);
+template <int M, int S, bool SD> int persistent_wgs_unit(int cus, int cp);
+
 template <int M, int S> void launch_batched_sd_cp(const BatchLaunch& B) {
-  if (B.sd && B.coupled) launch_batched_stage<M, S, true, true>(B);
-  else if (B.sd) launch_batched_stage<M, S, true, false>(B);
-  else if (B.coupled) launch_batched_stage<M, S, false, true>(B);
-  else launch_batched_stage<M, S, false, false>(B);
+  if (B.sd) launch_batched_unit<M, S, true>(B);
+  else launch_batched_unit<M, S, false>(B);
 }
 
 // the dt pass of the group's families, one launch each over every block
@@ -293,7 +298,7 @@ template <int M> void launch_batched_dt_kernel(const BatchLaunch& B) {
 }
 
 // persistent workgroups of the stage kernel on this device
-template <int M, int S, bool SD, bool CP> int persistent_wgs_of(int cus) {
+template <int M, int S, bool SD, int CP> int persistent_wgs_of(int cus) {
   int per = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_stage_persist<M, S, SD, CP, false, -1>,
                                                    kThreads, 0) != hipSuccess || per < 1)
@@ -308,9 +313,9 @@ void launch_batched_ag(const BatchLaunch& B);
 void launch_batched_dt_mk(const BatchLaunch& B);
 void launch_batched_dt_ig(const BatchLaunch& B);
 void launch_batched_dt_ag(const BatchLaunch& B);
-int persistent_wgs_mk(int cus);
-int persistent_wgs_ig(int cus);
-int persistent_wgs_ag(int cus);
+int persistent_wgs_mk(int cus, int cp);
+int persistent_wgs_ig(int cus, int cp);
+int persistent_wgs_ag(int cus, int cp);
 
 // defined in stage_mk.hip, stage_ig.hip, stage_ag.hip
 void launch_ghosts_mk(hipStream_t s, const int* grp, int ngrp, const FamArgs& a);

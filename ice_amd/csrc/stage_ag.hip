@@ -21,5 +21,5 @@ void launch_batched_ag(const BatchLaunch& B) {
   else launch_batched_sd_cp<2, 1>(B);
 }
 void launch_batched_dt_ag(const BatchLaunch& B) { launch_batched_dt_kernel<2>(B); }
-int persistent_wgs_ag(int cus) { return persistent_wgs_of<2, 1, true, true>(cus); }
+int persistent_wgs_ag(int cus, int cp) { return persistent_wgs_unit<2, 1, true>(cus, cp); }
 }  // namespace ice

@@ -21,5 +21,5 @@ void launch_batched_ig(const BatchLaunch& B) {
   else launch_batched_sd_cp<1, 1>(B);
 }
 void launch_batched_dt_ig(const BatchLaunch& B) { launch_batched_dt_kernel<1>(B); }
-int persistent_wgs_ig(int cus) { return persistent_wgs_of<1, 1, true, true>(cus); }
+int persistent_wgs_ig(int cus, int cp) { return persistent_wgs_unit<1, 1, true>(cus, cp); }
 }  // namespace ice

@@ -6,14 +6,18 @@
 namespace ice {
 
 constexpr int kThreads = 256;
-// Waves per SIMD the stage kernels of a closure are compiled for, i.e. their
-// register budget (512 VGPRs per SIMD lane shared by the resident waves).  Two
-// waves hide the latency of the face passes' loads and barriers for MK and IG,
-// whose stencils fit 256 registers; AG's 12-slot stencil spills at that budget
-// and runs faster alone.
-template <int M> struct StageWaves { static constexpr int value = 2; };
+// Waves per SIMD the stage kernels are compiled for, i.e. their register budget
+// (512 VGPRs per SIMD lane shared by the resident waves).  The stage waits on
+// its face passes' loads and barriers about half of its time, so two waves per
+// SIMD are the floor; the face pass keeps no stencil in registers (the limited
+// slopes come from LDS), so every closure, AG's 12 slots included, fits 256
+// registers.  ICE_MKIG_WAVES / ICE_AG_WAVES override it for measurements.
+#ifndef ICE_MKIG_WAVES
+#define ICE_MKIG_WAVES 2
+#endif
+template <int M> struct StageWaves { static constexpr int value = ICE_MKIG_WAVES; };
 #ifndef ICE_AG_WAVES
-#define ICE_AG_WAVES 1
+#define ICE_AG_WAVES 2
 #endif
 template <> struct StageWaves<2> { static constexpr int value = ICE_AG_WAVES; };
 
@@ -64,7 +68,6 @@ ICE_DEV float stage_seed(const FamArgs& a, bool has_cell, long long g, uint16_t*
   r = r < 1e-2f ? 1e-2f : (r > 1.0f ? 1.0f : r);
   return need ? r : 0.0f;
 }
-
 
 ICE_DEV void cell_of(const BlkDev& B, long long t, int& i, int& j, int& k) {
   i = (int)(t % B.ni) + 1;
@@ -574,15 +577,15 @@ ICE_DEV void update_cell(const FamArgs& a, const BlkDev& B, int i, int j, int k,
 // The end of one cell's stage from its flux sum r: the source (coupled, seeded by
 // seed), the residual (Lib_Residual.f90:147-156), the imports (add_births) and the
 // frozen slots, then the RK update, or the residual for smoothing (WRITE_R).
-template <int M, bool TAB, bool COUPLED, bool WRITE_R>
+template <int M, bool TAB, int CP, bool WRITE_R, bool LK = true>
 ICE_DEV void finish_cell(const FamArgs& a, const BlkDev& Bref, int ci, int cj, int ck, long long g, double* r,
                          float seed) {
   constexpr int NC = Closure<M>::NC;
   double src[NC];
-  if constexpr (COUPLED) {
+  if constexpr (CP != 0) {
     double q[NC], tau;
     load_cell<NC>(a.cur, a.N, g, q);
-    cell_source<M, TAB>(a, g, q, src, tau, seed);
+    cell_source<M, TAB, CP == 2, LK>(a, g, q, src, tau, seed);
     for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
     keep_stage_source<NC>(a, g, src, tau);
   } else {
@@ -614,8 +617,9 @@ ICE_DEV bool lk_cell_needed(const FamArgs& a, long long g) {
 }
 
 // Appends the wave's cells that need the interface solve to the family's queue (one
-// atomic per wave), with their block and flux sum; false for the others, and for
-// every cell past the queue's capacity (those finish where they are)
+// atomic per wave), with their block and flux sum; false for the others.  The
+// queue holds every interior cell of the family, so no cell can find it full;
+// one that did would be reported (lk_overflow) as an error of the step.
 template <int NC>
 ICE_DEV bool defer_cell(const FamArgs& a, bool need, long long g, int b, const double* r) {
   const unsigned long long mask = __ballot(need);
@@ -627,7 +631,10 @@ ICE_DEV bool defer_cell(const FamArgs& a, bool need, long long g, int b, const d
   base = (unsigned)__shfl((int)base, leader, 64);
   if (!need) return false;
   const unsigned slot = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-  if (slot >= (unsigned)a.lkq_cap) return false;
+  if (slot >= (unsigned)a.lkq_cap) {
+    atomicOr(&a.st->lk_overflow, 1);
+    return false;
+  }
   a.lkq_g[slot] = g;
   a.lkq_b[slot] = b;
 #pragma unroll
@@ -656,7 +663,7 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_lk_finish(Fa
       const BlkDev& B = a.blk[a.lkq_b[idx]];
       const long long loc = g - B.off;
       const int i = (int)(loc % B.NI) - 1, j = (int)((loc / B.NI) % B.NJ) - 1, k = (int)(loc / ((long long)B.NI * B.NJ)) - 1;
-      finish_cell<M, false, true, WRITE_R>(a, B, i, j, k, g, r, seed);
+      finish_cell<M, false, 2, WRITE_R>(a, B, i, j, k, g, r, seed);
     }
   }
 }
@@ -664,8 +671,9 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_lk_finish(Fa
 // The cell's stage: boundary sum, interior faces in the reference's order
 // (Mod_Fluxes.f90:471-508), source, residual (Lib_Residual.f90:147-156) and
 // either the RK update (WRITE_R = false) or the residual for smoothing.
-template <int M, int S, bool SD, bool COUPLED, bool WRITE_R>
+template <int M, int S, bool SD, int CP, bool WRITE_R>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage(FamArgs a, int b) {
+  constexpr bool COUPLED = CP != 0;
   constexpr int NC = Closure<M>::NC;
   const BlkDev& B = a.blk[b];
   const BlkDev Bv = B;
@@ -674,7 +682,7 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage(FamArg
   if (t < Bv.cells) cell_of(Bv, t, i, j, k);
   long long g = gidx(Bv, i, j, k);
   float seed = 0.0f;
-  if constexpr (COUPLED) {
+  if constexpr (CP == 2) {
     __shared__ __attribute__((aligned(16))) uint16_t mlds[kMlpLds];
     seed = stage_seed<M>(a, t < Bv.cells, g, mlds);
   }
@@ -711,7 +719,7 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage(FamArg
   if constexpr (COUPLED) {
     double q[NC], tau;
     load_cell<NC>(a.cur, a.N, g, q);
-    cell_source<M>(a, g, q, src, tau, seed);
+    cell_source<M, true, CP == 2>(a, g, q, src, tau, seed);
     for (int f = 0; f < a.nfrozen; ++f) src[a.frozen[f] - 1] = 0.0;
     keep_stage_source<NC>(a, g, src, tau);
   } else {
@@ -903,16 +911,127 @@ struct Tile {
 
 // slots of the face fluxes of one direction in LDS, structure of arrays
 constexpr int kMaxTileFaces = kThreads;
+// cells of a direction pass's extended box: the tile widened by one cell on each
+// side along the direction (every tile ice_gpu_set_tiles accepts fits)
+constexpr int kMaxExtCells = 352;
+// LDS of the stage tile, in doubles: the face fluxes of one direction (NC slots
+// of kMaxTileFaces faces), the limited slopes of the extended box (NC slots of
+// kMaxExtCells cells), the cells' half lengths and their first-order flags (bytes)
+template <int NC> constexpr int tile_lds_doubles() {
+  return NC * kMaxTileFaces + (NC + 1) * kMaxExtCells + kMaxExtCells / 8;
+}
+
+// The limited slope of every slot of one cell along direction d (padded index g,
+// index m of n along d, neighbours g -/+ s).  The MUSCL reconstruction
+// (Lib_Reconstruction.f90:17-78) of the face above the cell (its left state) calls
+// the limiter with (slope to the upper neighbour, slope to the lower neighbour,
+// cx, cy) over the mean spacings and the cell's half length, and that of the face
+// below it (its right state) with the same five values, so each cell's value is
+// formed once, with the same operations, and both faces read it.  The ghost
+// lengths an inflow face zeroes (Mod_Fluxes.f90:613-618) are those of the
+// neighbour past the first or the last cell.  first: the cell or a neighbour is
+// empty, and the cell's side of either face is first order.
+template <int M, int LIM>
+ICE_DEV void cell_slopes(const FamArgs& a, long long g, long long s, int d, int m, int n, double* slope, int e,
+                         double& half, bool& first) {
+  constexpr int NC = Closure<M>::NC;
+  int lim = a.limiter;
+  if constexpr (LIM >= 0) lim = LIM;
+  const long long N = a.N;
+  const double* len = a.dl + (long long)d * N;
+  double len0 = len[g - s], len1 = len[g], len2 = len[g + s];
+  if (m == 1 && a.inj[g - s]) len0 = 0.0;
+  if (m == n && a.inj[g + s]) len2 = 0.0;
+  const double dl0 = 0.5 * (len0 + len1), dl1 = 0.5 * (len1 + len2);
+  half = 0.5 * len1;
+  const double* p = a.cur;
+  first = p[g - s] <= kRhoEmpty || p[g] <= kRhoEmpty || p[g + s] <= kRhoEmpty;
+  if (first) return;
+  double cx, cy;
+  if (half > 0.0) { cx = dl1 / half; cy = dl0 / half; } else { cx = 2.0; cy = 2.0; }
+  const SharedDiv by0 = div_prepare(dl0), by1 = div_prepare(dl1);
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    const double* pv = p + (long long)v * N;
+    const double prev = pv[g - s], local = pv[g], next = pv[g + s];
+    const double s_up = div_shared(next - local, by1);
+    const double s_dn = div_shared(local - prev, by0);
+    slope[v * kMaxExtCells + e] = limiter(lim, s_up, s_dn, cx, cy);
+  }
+}
+
+// One interior face of a tile from its two cells' limited slopes (cell_slopes):
+// the left and right states with the reference's halving of the limiter factor
+// until both are realizable, or first order below 1e-18 (Lib_Reconstruction.f90:
+// 53-76), then the Riemann flux times the area (Mod_Fluxes.f90:655-685), into q.
+// g: the lower cell, s: the stride along d; elo/ehi: the cells' extended-box index.
+template <int M, int S, bool SD, bool TAB>
+ICE_DEV bool face_from_slopes(const FamArgs& a, long long g, long long s, int d, const double* slope,
+                              const double* half, const uint8_t* first, int elo, int ehi, double* q) {
+  constexpr int NC = Closure<M>::NC;
+  const long long N = a.N;
+  const double* p = a.cur;
+  const bool first_l = first[elo], first_r = first[ehi];
+  const double dll = half[elo], dlr = half[ehi];
+  const double bf = SD ? dmin2(a.beta[g], a.beta[g + s]) : 1.0;
+  double ql[NC], qr[NC];
+  bool ok = true;
+  if (first_l && first_r) {
+    load_cell<NC>(p, N, g, ql);
+    load_cell<NC>(p, N, g + s, qr);
+  } else {
+    double limval = bf;
+#pragma unroll
+    for (int v = 0; v < NC; ++v) {
+      const double lo = p[(long long)v * N + g], hi = p[(long long)v * N + g + s];
+      ql[v] = first_l ? lo : lo + slope[v * kMaxExtCells + elo] * limval * dll;
+      qr[v] = first_r ? hi : hi - slope[v * kMaxExtCells + ehi] * limval * dlr;
+    }
+    bool ok_l = first_l || checkprim<M>(ql), ok_r = first_r || checkprim<M>(qr);
+    if (__builtin_expect(!(ok_l && ok_r), 0)) {
+      // the reference's retries: every pass forms both sides from the halved factor
+      for (;;) {
+        limval = 0.5 * limval;
+        if (limval < 1.e-18) {
+          load_cell<NC>(p, N, g, ql);
+          load_cell<NC>(p, N, g + s, qr);
+          ok = checkprim<M>(ql) && checkprim<M>(qr);
+          break;
+        }
+#pragma unroll
+        for (int v = 0; v < NC; ++v) {
+          const double lo = p[(long long)v * N + g], hi = p[(long long)v * N + g + s];
+          if (!first_l) ql[v] = lo + slope[v * kMaxExtCells + elo] * limval * dll;
+          if (!first_r) qr[v] = hi - slope[v * kMaxExtCells + ehi] * limval * dlr;
+        }
+        if (!first_l) ok_l = checkprim<M>(ql);
+        if (!first_r) ok_r = checkprim<M>(qr);
+        if (ok_l && ok_r) break;
+      }
+    }
+  }
+  const double* fa = a.fa + (long long)(d * 4) * N;
+  const double nrm[3] = {fa[g], fa[N + g], fa[2 * N + g]};
+  riemann_acc<M, S, 2>(ql, qr, nrm, fa[3 * N + g], q, mat_of<TAB>(a));
+  return ok;
+}
+
 // The stage with the interior faces shared through LDS (Mod_Fluxes.f90:421-521:
 // each face once, each cell its two faces per direction in the odd-then-even
 // order), then the cell's source, residual and update as in k_stage.
 // One tile of the stage: the body of k_stage_tiled, also run by the persistent
-// kernel for the tiles it takes from its work queue.  fl is the workgroup's
-// LDS buffer of face fluxes, two halves of NC * kMaxTileFaces (alternate passes).
+// kernel for the tiles it takes from its work queue.  lds is the workgroup's
+// buffer of tile_lds_doubles<NC>() doubles.  Each direction pass: the limited
+// slopes of the extended box's cells, then each face once, then each cell sums
+// its two faces; two barriers a pass.
 // TAB = false: the launch serves only families without tabulated properties
-template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM, bool TAB = false>
-ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* fl, uint16_t* mlds) {
+template <int M, int S, bool SD, int CP, bool WRITE_R, int LIM, bool TAB = false>
+ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, double* lds) {
   constexpr int NC = Closure<M>::NC;
+  double* fp = lds;
+  double* slope = lds + NC * kMaxTileFaces;
+  double* half = slope + NC * kMaxExtCells;
+  uint8_t* first = (uint8_t*)(half + kMaxExtCells);
   const BlkDev& Bref = a.blk[b];
   const BlkDev B = Bref;
   int ox = (tile % T.ntx) * T.tx + 1;
@@ -935,29 +1054,41 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
   }
   bool ok = true;
   const int nd[3] = {B.ni, B.nj, B.nk};
-  const int td[3] = {T.tx, T.ty, T.tz};
-  // the faces of successive passes go to alternate halves of fl: a pass's faces
-  // are written after the barrier of the previous pass, which every wave reaches
-  // only when it has summed the faces of the pass before, so one barrier a pass
-  int pass = 0;
   for (int d = 0; d < 3; ++d) {
     int n = nd[d];
     if (n < 2) continue;
-    double* fp = fl + (pass & 1) * (NC * kMaxTileFaces);
-    ++pass;
-    // faces of the tile along d: lower-cell offset -1 .. t_d-1 along d
+    const long long stride = dir_stride(B, d);
+    // the extended box: offsets -1 .. t_d along d, the tile's extent across it
+    const int ex = T.tx + 2 * (d == 0), ey = T.ty + 2 * (d == 1), ez = T.tz + 2 * (d == 2);
+    const int ne = ex * ey * ez;
+    const int bx = ox - (d == 0), by = oy - (d == 1), bz = oz - (d == 2);
+    for (int e = t; e < ne; e += blockDim.x) {
+      const int li = bx + e % ex, lj = by + (e / ex) % ey, lk = bz + e / (ex * ey);
+      const int m = d == 0 ? li : (d == 1 ? lj : lk);
+      if (m >= 1 && m <= n && li <= B.ni && lj <= B.nj && lk <= B.nk) {
+        double h;
+        bool f1;
+        cell_slopes<M, LIM>(a, gidx(B, li, lj, lk), stride, d, m, n, slope, e, h, f1);
+        half[e] = h;
+        first[e] = f1;
+      }
+    }
+    __syncthreads();
+    // faces of the tile along d: lower-cell offset -1 .. t_d-1 along d; the face's
+    // lower cell has the same coordinates in the extended box, its upper the next
     int fx = T.tx + (d == 0), fy = T.ty + (d == 1), fz = T.tz + (d == 2);
     int nf = fx * fy * fz;
-    long long stride = dir_stride(B, d);
+    const int estep = d == 0 ? 1 : (d == 1 ? ex : ex * ey);
     for (int f = t; f < nf; f += blockDim.x) {
-      int li = ox + f % fx - (d == 0);
-      int lj = oy + (f / fx) % fy - (d == 1);
-      int lk = oz + f / (fx * fy) - (d == 2);
+      const int fi = f % fx, fj = (f / fx) % fy, fk = f / (fx * fy);
+      int li = bx + fi, lj = by + fj, lk = bz + fk;
       int m = d == 0 ? li : (d == 1 ? lj : lk);
       bool inside = m >= 1 && m <= n - 1 && li >= 1 && lj >= 1 && lk >= 1 && li <= B.ni && lj <= B.nj && lk <= B.nk;
       if (inside) {
+        const int elo = fi + ex * (fj + ey * fk);
         double q[NC];
-        ok = face_acc<M, S, SD, 2, LIM, TAB>(a, gidx(B, li, lj, lk), stride, d, m, n, q) && ok;
+        ok = face_from_slopes<M, S, SD, TAB>(a, gidx(B, li, lj, lk), stride, d, slope, half, first, elo,
+                                             elo + estep, q) && ok;
 #pragma unroll
         for (int v = 0; v < NC; ++v) fp[v * kMaxTileFaces + f] = q[v];
       }
@@ -990,20 +1121,18 @@ ICE_DEV void stage_tile_body(const FamArgs& a, int b, const Tile& T, int tile, d
   }
   if (!ok) atomicOr(&a.st->bad_recon, 1);
   // a cell whose source runs the Langmuir-Knudsen solve finishes in k_lk_finish,
-  // where such cells fill whole waves instead of holding up the rest of theirs
-  // (with the queue, only a cell past its capacity needs the seed here)
-  bool deferred = false, seeded = has_cell;
-  float seed = 0.0f;
-  if constexpr (COUPLED) {
+  // where such cells fill whole waves instead of holding up the rest of theirs;
+  // the solve is not compiled into the tile (a family with the solve always has
+  // the queue)
+  bool deferred = false;
+  if constexpr (CP == 2) {
     if (a.lkq_g) {
       const bool need = has_cell && lk_cell_needed<M, TAB>(a, g);
       deferred = defer_cell<NC>(a, need, g, b, r);
-      seeded = need && !deferred;
     }
-    seed = stage_seed<M, TAB>(a, seeded, g, mlds);
   }
   if (!has_cell || deferred) return;
-  finish_cell<M, TAB, COUPLED, WRITE_R>(a, Bref, ci, cj, ck, g, r, seed);
+  finish_cell<M, TAB, CP, WRITE_R, false>(a, Bref, ci, cj, ck, g, r, 0.0f);
 }
 
 // Workgroup w of a launch runs on XCD w % 8 (workgroups are dealt round-robin
@@ -1018,12 +1147,11 @@ ICE_DEV int xcd_tile(int w, int n) {
   return x * q + (x < r ? x : r) + j;
 }
 
-template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
+template <int M, int S, bool SD, int CP, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_tiled(FamArgs a, int b, Tile T) {
   constexpr int NC = Closure<M>::NC;
-  __shared__ double fl[2 * NC * kMaxTileFaces];
-  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
-  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(a, b, T, xcd_tile(blockIdx.x, gridDim.x), fl, mlds);
+  __shared__ double lds[tile_lds_doubles<NC>()];
+  stage_tile_body<M, S, SD, CP, WRITE_R, LIM>(a, b, T, xcd_tile(blockIdx.x, gridDim.x), lds);
 }
 
 // ------------------------------------------------------------------ batched path
@@ -1059,19 +1187,18 @@ ICE_DEV int next_item(const WorkQueues& Q, int q, int* shared_slot) {
   return it;
 }
 
-template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
+template <int M, int S, bool SD, int CP, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_persist(const FamArgs* __restrict__ fams,
                                                                                    WorkQueues Q) {
   constexpr int NC = Closure<M>::NC;
-  __shared__ double fl[2 * NC * kMaxTileFaces];
-  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
+  __shared__ double lds[tile_lds_doubles<NC>()];
   __shared__ int slot;
   int q = blockIdx.x % Q.nq;
   for (;;) {
     int it = next_item(Q, q, &slot);
     if (it < 0) break;
     const WorkItem w = Q.items[it];
-    stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
+    stage_tile_body<M, S, SD, CP, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, lds);
   }
 }
 
@@ -1079,15 +1206,14 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_persis
 // workgroups: the hardware dispatcher balances the items, dealt to the XCDs in
 // contiguous runs (an item's tile neighbours and its sibling families share
 // that XCD's L2).  Same tiles, same bits as k_stage_persist.
-template <int M, int S, bool SD, bool COUPLED, bool WRITE_R, int LIM>
+template <int M, int S, bool SD, int CP, bool WRITE_R, int LIM>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage_items(const FamArgs* __restrict__ fams,
                                                                                  const WorkItem* __restrict__ items) {
   constexpr int NC = Closure<M>::NC;
-  __shared__ double fl[2 * NC * kMaxTileFaces];
-  __shared__ __attribute__((aligned(16))) uint16_t mlds[COUPLED ? kMlpLds : 8];
+  __shared__ double lds[tile_lds_doubles<NC>()];
   const int it = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x));
   const WorkItem w = items[it];
-  stage_tile_body<M, S, SD, COUPLED, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, fl, mlds);
+  stage_tile_body<M, S, SD, CP, WRITE_R, LIM>(fams[w.fam], w.block, w.T, w.tile, lds);
 }
 
 // the sensor weights over the same queues, one cell per thread

@@ -23,5 +23,5 @@ void launch_batched_mk(const BatchLaunch& B) {
   else launch_batched_sd_cp<0, 2>(B);
 }
 void launch_batched_dt_mk(const BatchLaunch& B) { launch_batched_dt_kernel<0>(B); }
-int persistent_wgs_mk(int cus) { return persistent_wgs_of<0, 1, true, true>(cus); }
+int persistent_wgs_mk(int cus, int cp) { return persistent_wgs_unit<0, 1, true>(cus, cp); }
 }  // namespace ice

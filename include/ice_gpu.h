@@ -181,7 +181,8 @@ int  ice_gpu_residual_norm_noalloc(ice_gpu_ctx* ctx, double* total, double* fam)
  * the summed duration and launch count of the dominant kernel in it. */
 int  ice_gpu_timing(ice_gpu_ctx* ctx, double* step_ms, double* kernel_ms, int64_t* kernel_launches);
 /* Cells per workgroup tile of the stage kernel (tx*ty*tz, and the faces of the
- * tile in each direction, at most 256); tx = -1 restores the default shapes,
+ * tile in each direction, at most 256; the tile widened by one cell along each
+ * resolved direction, at most 352); tx = -1 restores the default shapes,
  * tx = 0 selects the untiled one-thread-per-cell kernel. */
 int  ice_gpu_set_tiles(ice_gpu_ctx* ctx, int tx, int ty, int tz);
 /* Dispatch of the batched stage kernel: 0 persistent workgroups over per-XCD tile
