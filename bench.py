@@ -143,17 +143,16 @@ def cpu_baseline(case, max_steps: int, budget_s: float = 12.0):
     return case.cell_stages(n) / dt / 1e6, dt, n
 
 
-def traffic_of(workload: str, mode: str, cells_per_launch: float):
-    """HBM bytes per launch of the stage kernel from the committed rocprofv3 PMC
-    summary of this workload (profiles/traffic_<workload>_<mode>.json: FETCH_SIZE
-    and WRITE_SIZE per launch, scaled by the factors a 1 GiB copy gives), scaled to
-    this run's cells per launch; None when no summary exists."""
+def counters_of(workload: str, mode: str):
+    """The committed rocprofv3 PMC summary of this workload's stage kernel
+    (profiles/traffic_<workload>_<mode>.json: HBM bytes per cell-stage from
+    FETCH_SIZE and WRITE_SIZE, scaled by the factors a 1 GiB copy gives, and FP64
+    flops per cell-stage from the SQ_INSTS_VALU_*_F64 counters), or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{mode}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        t = json.load(fh)
-    return t["bytes_per_cell"] * cells_per_launch
+        return json.load(fh)
 
 
 def autotune(gpu, run, steps: int = 2):
@@ -176,6 +175,20 @@ def device_used_gb():
     if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
         return None
     return (total.value - free.value) / 1e9
+
+
+def kernel_pass(gpu, run, steps: int = 1):
+    """`steps` steps with HIP events around every launch of the stage kernel and of
+    the deferred interface-solve kernel: (stage ms, stage launches, solve ms, solve
+    launches) summed over the steps."""
+    gpu.ice_gpu_set_kernel_timing(run.ctx, 1)
+    run.step(steps)
+    gpu.ice_gpu_set_kernel_timing(run.ctx, 0)
+    km, nl = C.c_double(), C.c_int64()
+    gpu.ice_gpu_timing(run.ctx, None, C.byref(km), C.byref(nl))
+    lm, ll = C.c_double(), C.c_int64()
+    gpu.ice_gpu_lk_timing(run.ctx, C.byref(lm), C.byref(ll))
+    return km.value, nl.value, lm.value, ll.value
 
 
 def timed_steps(run, steps: int) -> float:
@@ -214,16 +227,17 @@ def extra_runs(gpu, steps: int, progress):
     tuned = autotune(gpu, run)
     run.step(1)
     wall = timed_steps(run, steps)
-    gpu.ice_gpu_set_kernel_timing(run.ctx, 1)
-    run.step(1)
-    gpu.ice_gpu_set_kernel_timing(run.ctx, 0)
-    km, nl = C.c_double(), C.c_int64()
-    gpu.ice_gpu_timing(run.ctx, None, C.byref(km), C.byref(nl))
-    avg_s = km.value / max(nl.value, 1) / 1e3
+    km, nl, _, _ = kernel_pass(gpu, run)
+    avg_s = km / max(nl, 1) / 1e3
     bpc = stage_kernel_bytes(case, True)
-    cpl = case.cell_stages(1) / max(nl.value, 1)
+    cpl = case.cell_stages(1) / max(nl, 1)
     ach = cpl * bpc / avg_s / 1e9 if avg_s > 0 else None
-    out["multiblock3d_batched"] = {"workload": DESCR["multiblock3d"], "steps": steps,
+    ctr = counters_of("multiblock3d", "batched")
+    fp64 = None
+    if ctr and ctr.get("fp64_flop_per_cell") and avg_s > 0:
+        fp64 = ctr["fp64_flop_per_cell"] * cpl / avg_s / 1e12
+    out["multiblock3d_batched"] = {"workload": DESCR["multiblock3d"], "steps": steps, "fp64_tflops": fp64,
+                                   "traffic_bytes_per_cell": ctr["bytes_per_cell"] if ctr else None,
                                    "value": case.cell_stages(steps) / wall / 1e6, "autotune": tuned,
                                    "kernel_avg_ms": avg_s * 1e3, "bytes_per_cell": bpc, "achieved_gbs": ach,
                                    "frac": ach / HBM_PEAK_GBS if ach else None}
@@ -241,11 +255,16 @@ def extra_runs(gpu, steps: int, progress):
                 raise RuntimeError(run.error())
             ev["training"] = {"samples": int(rep[0]), "loss_final": float(rep[2]),
                               "rate_ratio_abs_err_mean": float(rep[3]), "train_ms": float(rep[5])}
-        run.step(1)
+        # the first step from the shared initial state: the same cells need the solve
+        # in every mode, so the deferred-solve kernel's time compares the solves
+        _, _, lm, ll = kernel_pass(gpu, run)
+        ev[f"{label}_solve_kernel_ms_first_step"] = lm
         ev[f"{label}_value"] = case.cell_stages(steps) / timed_steps(run, steps) / 1e6
         ev[f"{label}_lk"] = lk_stats(gpu, run, case)[0]
         run.close()
     ev["speedup_network_vs_plain"] = ev["network_value"] / ev["plain_value"]
+    ev["solve_kernel_speedup_network_vs_plain"] = (ev["plain_solve_kernel_ms_first_step"] /
+                                                   max(ev["network_solve_kernel_ms_first_step"], 1e-9))
     ev["workload"] = DESCR["evap3d"]
     out["evap3d_surrogate"] = ev
     progress("evaporation block done")
@@ -354,15 +373,11 @@ def main():
     progress("timed region done")
     # the dominant kernel, timed live with HIP events on the context's stream
     # (bracketing each launch; a separate pass so the brackets do not slow the timed region)
-    gpu.ice_gpu_set_kernel_timing(run.ctx, 1)
-    run.step(args.kernel_steps)
-    gpu.ice_gpu_set_kernel_timing(run.ctx, 0)
-    kern_ms, launches = C.c_double(), C.c_int64()
-    gpu.ice_gpu_timing(run.ctx, None, C.byref(kern_ms), C.byref(launches))
+    kern_ms, launches, lk_ms, lk_launches = kernel_pass(gpu, run, args.kernel_steps)
     cell_stages = case.cell_stages(args.steps)
     value = cell_stages / wall / 1e6
-    avg_kernel_s = kern_ms.value / max(launches.value, 1) / 1e3
-    cells_per_launch = case.cell_stages(args.kernel_steps) / max(launches.value, 1)
+    avg_kernel_s = kern_ms / max(launches, 1) / 1e3
+    cells_per_launch = case.cell_stages(args.kernel_steps) / max(launches, 1)
     bpc = stage_kernel_bytes(case, batched)
     bytes_per_launch = cells_per_launch * bpc
     achieved = bytes_per_launch / avg_kernel_s / 1e9 if avg_kernel_s > 0 else None
@@ -384,8 +399,10 @@ def main():
             gpu.ice_gpu_set_mode(run.ctx, m)
             run.step(1)   # the graph of the new arguments, or the plain path's first launches
             w = timed_steps(run, args.compare_steps)
+            _, _, lm, ll = kernel_pass(gpu, run)
             compared.append({"mode": label, "steps": args.compare_steps,
-                             "value": case.cell_stages(args.compare_steps) / w / 1e6})
+                             "value": case.cell_stages(args.compare_steps) / w / 1e6,
+                             "solve_kernel_ms_per_step": lm if ll else None})
         progress("comparison runs done")
     it1 = C.c_int32()
     gpu.ice_gpu_get_counters(run.ctx, C.byref(it1), None, None)
@@ -428,12 +445,14 @@ def main():
                    "mode": args.mode + ("+surrogate" if seeded else ""), "cells": case.cells,
                    "families": case.cfg.ngroups, "rk_stages": case.cfg.nrk, "device_gb_in_use": used_gb,
                    "event_ms_per_step": step_event_ms / args.steps, "iter_before": it0.value, "iter_after": it1.value,
-                   "autotune": tuned, "compared": compared, "interface_solve": lk},
+                   "autotune": tuned, "compared": compared, "interface_solve": lk,
+                   "solve_kernel_ms_per_step": (lk_ms / args.kernel_steps) if lk_launches else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                      "kernel": "k_stage_persist / k_stage_items / k_stage_tiled (fused interior flux + source + "
-                               "residual, and the RK update where there is no IRS)",
-                     "kernel_avg_ms": avg_kernel_s * 1e3, "bytes_per_cell": bpc, "launches": launches.value,
+                               "residual, and the RK update where there is no IRS; the cells that run the interface "
+                               "solve finish in k_lk_finish, timed apart)",
+                     "kernel_avg_ms": avg_kernel_s * 1e3, "bytes_per_cell": bpc, "launches": launches,
                      "path_bytes_per_cell": path_bytes(case),
                      "path_frac": value * 1e6 * path_bytes(case) / (HBM_PEAK_GBS * 1e9)},
         "cpu_baseline": cpu,
@@ -442,9 +461,13 @@ def main():
         out["surrogate"] = surrogate
     if extras is not None:
         out["extras"] = extras
-    traffic = traffic_of(args.workload, args.mode, cells_per_launch)
-    if traffic is not None:
-        out["roofline"]["traffic"] = traffic
+    ctr = counters_of(args.workload, args.mode)
+    if ctr is not None:
+        out["roofline"]["traffic"] = ctr["bytes_per_cell"] * cells_per_launch
+        if ctr.get("fp64_flop_per_cell") and avg_kernel_s > 0:
+            tf = ctr["fp64_flop_per_cell"] * cells_per_launch / avg_kernel_s / 1e12
+            out["roofline"]["fp64"] = {"achieved": tf, "peak": 78.6, "unit": "TFLOP/s", "frac": tf / 78.6,
+                                       "flop_per_cell": ctr["fp64_flop_per_cell"]}
     print(json.dumps(out))
 
 

@@ -103,9 +103,10 @@ struct ice_gpu_ctx {
   int keep_source = 0;
   int kernel_timing = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double step_ms = 0.0, kernel_ms = 0.0;
-  long long kernel_launches = 0;
+  double step_ms = 0.0, kernel_ms = 0.0, lk_ms = 0.0;
+  long long kernel_launches = 0, lk_launches = 0;
   std::vector<hipEvent_t> kev;
+  std::vector<hipEvent_t> lkev;    // brackets of the deferred interface-solve launches (kernel timing)
   std::vector<FamArgs*> kev_args;  // FamArgs arrays of eager batched steps, freed with the events
   BatchedPlan* batched = nullptr;
   std::vector<SurrDev> surr;   // per family
@@ -533,7 +534,7 @@ void dispatch_stage(ice_gpu_ctx* c, FamArgs& a) {
   L.irs = c->cfg.irs; L.limiter = c->cfg.limiter; L.ndir = c->cfg.ndir;
   L.irs_wg = c->d_irs_wg; L.irs_nwg = c->irs_nwg;
   for (int d = 0; d < 2; ++d) { L.irs_wl[d] = c->d_irs_wl[d]; L.irs_nwl[d] = c->irs_nwl[d]; }
-  L.timing = c->kernel_timing; L.kev = &c->kev;
+  L.timing = c->kernel_timing; L.kev = &c->kev; L.lkev = &c->lkev;
   L.lk_grid = c->lk_grid;
   int cl = c->fams[a.fam].closure;
   if (cl == ICE_MK) launch_stage_mk(L, a);
@@ -968,6 +969,8 @@ int ice_gpu_step(ice_gpu_ctx* c, int nsteps, ice_status_t* st) {
   if (c->rec.empty()) return fail(c, "[ICE::step] no boundary table: call ice_gpu_set_boundaries first");
   for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
   c->kev.clear();
+  for (hipEvent_t e : c->lkev) (void)hipEventDestroy(e);
+  c->lkev.clear();
   for (FamArgs* d : c->kev_args) (void)hipFree(d);
   c->kev_args.clear();
   HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -1006,6 +1009,14 @@ int ice_gpu_step(ice_gpu_ctx* c, int nsteps, ice_status_t* st) {
     (void)hipEventElapsedTime(&km, c->kev[m], c->kev[m + 1]);
     c->kernel_ms += km;
     c->kernel_launches += 1;
+  }
+  c->lk_ms = 0.0;
+  c->lk_launches = 0;
+  for (size_t m = 0; m + 1 < c->lkev.size(); m += 2) {
+    float km = 0.f;
+    (void)hipEventElapsedTime(&km, c->lkev[m], c->lkev[m + 1]);
+    c->lk_ms += km;
+    c->lk_launches += 1;
   }
   DevStatus s;
   HIPCHK(hipMemcpy(&s, c->d_st, sizeof s, hipMemcpyDeviceToHost));
@@ -1119,6 +1130,12 @@ int ice_gpu_timing(ice_gpu_ctx* c, double* step_ms, double* kernel_ms, int64_t* 
   if (step_ms) *step_ms = c->step_ms;
   if (kernel_ms) *kernel_ms = c->kernel_ms;
   if (launches) *launches = c->kernel_launches;
+  return 0;
+}
+
+int ice_gpu_lk_timing(ice_gpu_ctx* c, double* ms, int64_t* launches) {
+  if (ms) *ms = c->lk_ms;
+  if (launches) *launches = c->lk_launches;
   return 0;
 }
 
@@ -1601,6 +1618,7 @@ void ice_gpu_destroy(ice_gpu_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->lkev) (void)hipEventDestroy(e);
   auto F = [](void* p) {
     if (p) (void)hipFree(p);
   };
@@ -1827,6 +1845,7 @@ void launch_batched_step(ice_gpu_ctx* c, const StepArgs& A, bool timed) {
       if (timed) {
         (void)hipEventCreate(&B.ev0);
         (void)hipEventCreate(&B.ev1);
+        (void)hipEventCreate(&B.ev2);
       }
       if (g.closure == ICE_MK) launch_batched_mk(B);
       else if (g.closure == ICE_IG) launch_batched_ig(B);
@@ -1834,6 +1853,8 @@ void launch_batched_step(ice_gpu_ctx* c, const StepArgs& A, bool timed) {
       if (timed) {
         c->kev.push_back(B.ev0);
         c->kev.push_back(B.ev1);
+        c->lkev.push_back(B.ev1);
+        c->lkev.push_back(B.ev2);
       }
     }
   }

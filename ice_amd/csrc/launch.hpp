@@ -27,6 +27,7 @@ struct StageLaunch {
   int irs_nwl[2];
   int timing;                        // bracket each stage launch with events
   std::vector<hipEvent_t>* kev;
+  std::vector<hipEvent_t>* lkev;     // and each deferred interface-solve launch
   int lk_grid;                       // workgroups of k_lk_finish (its grid-stride loop)
 };
 
@@ -101,7 +102,7 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
   if (CP == 2) clear_lk_queue(a, L.stream);
   for (int b = 0; b < nb; ++b) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    bool timed = L.timing && !L.irs;
+    bool timed = L.timing;
     if (timed) {
       (void)hipEventCreate(&e0);
       (void)hipEventCreate(&e1);
@@ -124,20 +125,18 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
     }
   }
   if (CP == 2 && a.lkq_g) {
-    // bracketed as one more launch of the stage's kernels when timing
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    bool timed = L.timing && !L.irs;
-    if (timed) {
+    if (L.timing) {
       (void)hipEventCreate(&e0);
       (void)hipEventCreate(&e1);
       (void)hipEventRecord(e0, L.stream);
     }
     if (L.irs) launch_lk_finish<M, true>(a, L.lk_grid, L.stream);
     else launch_lk_finish<M, false>(a, L.lk_grid, L.stream);
-    if (timed) {
+    if (L.timing) {
       (void)hipEventRecord(e1, L.stream);
-      L.kev->push_back(e0);
-      L.kev->push_back(e1);
+      L.lkev->push_back(e0);
+      L.lkev->push_back(e1);
     }
   }
   if (L.irs) {
@@ -200,7 +199,9 @@ struct BatchLaunch {
   int riemann, sd, limiter;
   int coupled;                 // 0 uncoupled, 1 coupled, 2 coupled with an evaporating family in the group
   int lk_grid;                 // workgroups of k_lk_finish
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // recorded around the persistent stage kernel when set
+  // recorded before the stage kernel, after it and after the deferred interface
+  // solves when set (kernel timing)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 };
 
 // the deferred interface solves of the group's families, after the stage kernel
@@ -258,8 +259,9 @@ template <int M, int S, bool SD, int CP> void launch_batched_stage(const BatchLa
       hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, true, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q);
     else
       hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, true, -1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
-    finish_lk_group<M, CP>(B, true);
     if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
+    finish_lk_group<M, CP>(B, true);
+    if (B.ev2) (void)hipEventRecord(B.ev2, B.stream);
     constexpr int NC = Closure<M>::NC;
     for (int f = 0; f < B.nfam; ++f) {
       FamArgs a = B.h_fams[B.fam_ids[f]];
@@ -279,8 +281,9 @@ template <int M, int S, bool SD, int CP> void launch_batched_stage(const BatchLa
     else if (B.limiter == 3) hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, 3>), grid, blk, 0, B.stream, B.d_fams, B.Q);
     else hipLaunchKernelGGL((k_stage_persist<M, S, SD, CP, false, -1>), grid, blk, 0, B.stream, B.d_fams, B.Q);
   }
-  finish_lk_group<M, CP>(B, false);
   if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
+  finish_lk_group<M, CP>(B, false);
+  if (B.ev2) (void)hipEventRecord(B.ev2, B.stream);
 }
 
 template <int M, int S, bool SD> void launch_batched_unit(const BatchLaunch& B);

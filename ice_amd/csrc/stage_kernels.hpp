@@ -913,7 +913,10 @@ struct Tile {
 constexpr int kMaxTileFaces = kThreads;
 // cells of a direction pass's extended box: the tile widened by one cell on each
 // side along the direction (every tile ice_gpu_set_tiles accepts fits)
-constexpr int kMaxExtCells = 352;
+#ifndef ICE_MAX_EXT_CELLS
+#define ICE_MAX_EXT_CELLS 352
+#endif
+constexpr int kMaxExtCells = ICE_MAX_EXT_CELLS;
 // LDS of the stage tile, in doubles: the face fluxes of one direction (NC slots
 // of kMaxTileFaces faces), the limited slopes of the extended box (NC slots of
 // kMaxExtCells cells), the cells' half lengths and their first-order flags (bytes)

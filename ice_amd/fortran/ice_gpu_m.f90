@@ -69,7 +69,7 @@ module ice_gpu_m
             ice_gpu_surrogate_params, ice_gpu_surrogate_set_params, ice_gpu_set_lk_stats, ice_gpu_lk_stats, &
             ice_gpu_set_mode, ice_gpu_set_surrogate, ice_gpu_set_counters, ice_gpu_get_counters, &
             ice_gpu_fill_ghosts, ice_gpu_put_imports, ice_gpu_add_births, ice_gpu_get_tau, ice_gpu_surrogate_retrain, &
-            ice_gpu_residual_norm_noalloc
+            ice_gpu_residual_norm_noalloc, ice_gpu_lk_timing
   integer, parameter, public :: ICE_MLP_PARAMS = 6288
 
   interface
@@ -135,6 +135,12 @@ module ice_gpu_m
       import :: c_int, c_double, c_int64_t, c_ptr
       type(c_ptr), value :: ctx
       real(c_double) :: step_ms, kernel_ms
+      integer(c_int64_t) :: launches
+    end function
+    integer(c_int) function ice_gpu_lk_timing(ctx, ms, launches) bind(C, name='ice_gpu_lk_timing')
+      import :: c_int, c_double, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: ms
       integer(c_int64_t) :: launches
     end function
     integer(c_int) function ice_gpu_autotune(ctx, steps, best, ms) bind(C, name='ice_gpu_autotune')

@@ -93,6 +93,7 @@ SIGNATURES = {
     "ice_gpu_residual_norm_noalloc": [C.c_void_p, _DP, _DP],
     "ice_gpu_timing": [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "ice_gpu_set_kernel_timing": [C.c_void_p, C.c_int],
+    "ice_gpu_lk_timing": [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "ice_gpu_set_tiles": [C.c_void_p, C.c_int, C.c_int, C.c_int],
     "ice_gpu_set_dispatch": [C.c_void_p, C.c_int],
     "ice_gpu_autotune": [C.c_void_p, C.c_int, _IP, np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")],

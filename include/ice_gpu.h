@@ -178,8 +178,14 @@ int  ice_gpu_residual_norm(ice_gpu_ctx* ctx, double* total, double* fam);
 int  ice_gpu_residual_norm_noalloc(ice_gpu_ctx* ctx, double* total, double* fam);
 
 /* Time (ms, HIP events on the context's stream) of the last ice_gpu_step, and
- * the summed duration and launch count of the dominant kernel in it. */
+ * the summed duration and launch count of the dominant kernel in it (the fused
+ * stage kernel; with ice_gpu_set_kernel_timing). */
 int  ice_gpu_timing(ice_gpu_ctx* ctx, double* step_ms, double* kernel_ms, int64_t* kernel_launches);
+/* The summed duration and launch count of the deferred interface-solve kernel
+ * (k_lk_finish: the cells that run the Langmuir-Knudsen solve, their source,
+ * residual and update) in the last ice_gpu_step run with kernel timing on.  No
+ * counterpart in the reference: it measures the solve the learned seed speeds up. */
+int  ice_gpu_lk_timing(ice_gpu_ctx* ctx, double* ms, int64_t* launches);
 /* Cells per workgroup tile of the stage kernel (tx*ty*tz, and the faces of the
  * tile in each direction, at most 256; the tile widened by one cell along each
  * resolved direction, at most 352); tx = -1 restores the default shapes,

@@ -36,7 +36,7 @@ def test_oracle_exports_same_api():
     exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
     # device-only: timing, tiles, the on-GPU training of the learned seed and its solve counters,
     # and the residual norm on the device's scratch array
-    skip = {"ice_gpu_timing", "ice_gpu_set_kernel_timing", "ice_gpu_set_tiles", "ice_gpu_set_keep_source",
+    skip = {"ice_gpu_timing", "ice_gpu_set_kernel_timing", "ice_gpu_lk_timing", "ice_gpu_set_tiles", "ice_gpu_set_keep_source",
             "ice_gpu_autotune", "ice_gpu_set_dispatch",
             "ice_gpu_surrogate_train", "ice_gpu_surrogate_params", "ice_gpu_surrogate_retrain_every", "ice_gpu_set_lk_stats", "ice_gpu_lk_stats",
             "ice_gpu_surrogate_retrain", "ice_gpu_residual_norm_noalloc"}
