@@ -139,7 +139,8 @@ def read_bc(path: str, dims: Sequence[Tuple[int, int, int]], ngroups: int):
         if pos >= len(lines):
             raise FormatError(f"{path}: the file ends before the last boundary cell")
         pos += 1
-        return pos, lines[pos - 1].split()
+        # list-directed input (IO_BC.f90:261-265): commas separate values as blanks do
+        return pos, [t for t in re.split(r"[,\s]+", lines[pos - 1].strip()) if t]
 
     ncell = sum(2 * (nj * nk + ni * nk + ni * nj) for ni, nj, nk in dims)
     ri = np.zeros((ncell, REC_NI), np.int32)

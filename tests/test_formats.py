@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from ice_amd import formats
-from ice_amd.native import R_DONOR0, R_NCHI1, R_NCHI2, R_NORMAL_DIR, R_TYPE, RD_ALPHA, RD_MASSFLUX
+from ice_amd.native import (R_DONOR0, R_NCHI1, R_NCHI2, R_NORMAL_DIR, R_TYPE, RD_ALPHA, RD_MASSFLUX, RD_RADIUS,
+                            RD_TEMPERATURE)
 
 from helpers import GOLDEN
 
@@ -66,6 +67,19 @@ def test_bc_payloads(tmp_path):
     c = [n for n in range(reci.shape[0]) if reci[n, R_TYPE] == 102]
     assert len(c) == 2 and reci[c[0], R_NCHI1] == 1 and reci[c[0], R_NCHI2] == 2 and reci[c[0], R_DONOR0] == 0
     assert donor.tolist() == [[1, 1, 1, 1], [1, 2, 1, 1], [1, 2, 2, 1]] and weight.tolist() == [1.0, 0.25, 0.75]
+
+
+def test_bc_payload_commas(tmp_path):
+    """The payload is read list-directed (IO_BC.f90:261-265), where a comma
+    separates values as a blank does: ICE's Axis/MK case writes 'normal, normal,'."""
+    dims = (3, 2, 1)
+    special = {
+        (1, 1, 1, 1): lambda i, j, k, f: [f"1 {i} {j} {k} {f} 402", "0.5 10.0 normal, normal, 300.0 1e-6"],
+        (1, 1, 2, 1): lambda i, j, k, f: [f"1 {i} {j} {k} {f} 402", "0.5,10.0,0.3,0.0,300.0,1e-6"],
+    }
+    reci, recd, _, _ = formats.read_bc(bc_file(tmp_path, write_block(dims, special)), [dims], 1)
+    assert reci[0, R_NORMAL_DIR] == 1 and recd[0, RD_MASSFLUX] == 0.5 and recd[0, RD_RADIUS] == 1e-6
+    assert reci[1, R_NORMAL_DIR] == 0 and recd[1, RD_ALPHA] == 0.3 and recd[1, RD_TEMPERATURE] == 300.0
 
 
 @pytest.mark.parametrize("bad, msg", [
