@@ -14,7 +14,16 @@ case's verify.py:
 * Reflect AG (verify.py:30-41): RK3 to t = 0.1, the x-momentum sum(rho u V)
   changes by less than 1e-9 relative;
 * Thermal IG, AG (verify.py:35-50): RK3 to t = 0.1, max|T - 1| < 1e-9 over the
-  cells with rho > 1e-3, and the blob has expanded (max|u| >= 0.1).
+  cells with rho > 1e-3, and the blob has expanded (max|u| >= 0.1);
+* Riemann AG (verify.py:70-100): an anisotropic Sod tube, explicit Euler at first
+  order with Rusanov, against the exact gamma = 3 Riemann solution (Toro ch. 4,
+  restated here and checked on Toro's Sod values): min(P11) > 1e3, L1 density
+  error <= 0.05, P22/rho kept to 1e-9, total variation of rho and P11 at most
+  1.05 times the exact one;
+* Berthon SCS, RCS, RCR, SCS-hlle (berthon.py:87-110): AG Riemann problems to
+  t = 0.125 against the exact profiles the reference tabulates, the L1 error of
+  rho, u, v, P11, P22 and det(P) over the span of each exact profile within
+  2.5e-2 (SCS) or 1.5e-2 (RCS, RCR).
 
 The wedge is the curvilinear mesh of these cases: its metrics are also held to
 their closed forms (a prism per cell), and the volume formula to the divergence
@@ -99,22 +108,143 @@ def check_thermal(q, closure):
     assert np.isfinite(q).all() and umax >= 0.1 and dT < 1e-9
 
 
+def exact_riemann(WL, WR, gam):
+    """The exact Riemann solution of the Euler equations, W = (rho, u, p) (Toro,
+    Riemann Solvers, ch. 4): Newton on f_L(p) + f_R(p) + u_R - u_L = 0, then the
+    wave pattern sampled at S = (x - x0)/t.  Returns (p*, u*, sample)."""
+    rL, uL, pL = WL
+    rR, uR, pR = WR
+    cL, cR = np.sqrt(gam * pL / rL), np.sqrt(gam * pR / rR)
+
+    def f(p, r, pk, c):
+        if p > pk:  # shock
+            A, B = 2.0 / ((gam + 1.0) * r), (gam - 1.0) / (gam + 1.0) * pk
+            q = np.sqrt(A / (p + B))
+            return (p - pk) * q, q * (1.0 - 0.5 * (p - pk) / (p + B))
+        e = (gam - 1.0) / (2.0 * gam)  # rarefaction
+        return 2.0 * c / (gam - 1.0) * ((p / pk) ** e - 1.0), (p / pk) ** (-(gam + 1.0) / (2.0 * gam)) / (r * c)
+
+    p = 0.5 * (pL + pR)
+    for _ in range(100):
+        fl, dl = f(p, rL, pL, cL)
+        fr, dr = f(p, rR, pR, cR)
+        pn = max(p - (fl + fr + uR - uL) / (dl + dr), 1e-12 * p)
+        if abs(pn - p) <= 1e-14 * p:
+            p = pn
+            break
+        p = pn
+    us = 0.5 * (uL + uR) + 0.5 * (f(p, rR, pR, cR)[0] - f(p, rL, pL, cL)[0])
+    g = gam
+
+    def side(S, r, u, pk, c, sgn):
+        """the state at S on the side of sign sgn (-1 left, +1 right) of the contact"""
+        if p > pk:  # shock
+            Ssh = u + sgn * c * np.sqrt((g + 1) / (2 * g) * p / pk + (g - 1) / (2 * g))
+            if sgn * (S - Ssh) >= 0:
+                return r, u, pk
+            q = p / pk
+            return r * (q + (g - 1) / (g + 1)) / ((g - 1) / (g + 1) * q + 1), us, p
+        cs = c * (p / pk) ** ((g - 1) / (2 * g))
+        head, tail = u + sgn * c, us + sgn * cs
+        if sgn * (S - head) >= 0:
+            return r, u, pk
+        if sgn * (S - tail) <= 0:
+            return r * (p / pk) ** (1 / g), us, p
+        b = 2 / (g + 1) - sgn * (g - 1) / ((g + 1) * c) * (u - S)
+        return r * b ** (2 / (g - 1)), 2 / (g + 1) * (-sgn * c + (g - 1) / 2 * u + S), pk * b ** (2 * g / (g - 1))
+
+    def sample(S):
+        return side(S, rL, uL, pL, cL, -1) if S <= us else side(S, rR, uR, pR, cR, 1)
+
+    return p, us, sample
+
+
+def test_exact_riemann_solver_on_toro_sod():
+    """Toro's Sod problem (Table 4.3, gamma = 1.4): p* = 0.30313, u* = 0.92745;
+    and with equal states the solution is that state everywhere."""
+    ps, us, sample = exact_riemann((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4)
+    assert abs(ps - 0.30313) < 1e-5 and abs(us - 0.92745) < 1e-5
+    assert np.allclose(sample(-10.0), (1.0, 0.0, 1.0)) and np.allclose(sample(10.0), (0.125, 0.0, 0.1))
+    ps, us, sample = exact_riemann((2.0, 0.5, 3.0), (2.0, 0.5, 3.0), 3.0)
+    assert abs(ps - 3.0) < 1e-12 and abs(us - 0.5) < 1e-12 and np.allclose(sample(0.3), (2.0, 0.5, 3.0))
+
+
+def cell_x(d):
+    x = d["nodes"][0][:, 0, 0]
+    return 0.5 * (x[:-1] + x[1:])
+
+
+def check_riemann_ag(q, d, t):
+    ic = d["ic"][:, :, :, 0]
+    xc = cell_x(d)
+    jump = int(np.argmax(np.abs(np.diff(ic[0][:, 0]))))
+    x0 = d["nodes"][0][jump + 1, 0, 0]
+    WL, WR = (ic[0][0, 0], 0.0, ic[4][0, 0]), (ic[0][-1, 0], 0.0, ic[4][-1, 0])
+    ratio = float(ic[7][0, 0] / ic[0][0, 0])
+    _, _, sample = exact_riemann(WL, WR, 3.0)
+    ex = np.array([sample((x - x0) / t) for x in xc])
+    rho, P11, P22 = q[0], q[4], q[7]
+    l1 = float(np.abs(rho.mean(axis=1) - ex[:, 0]).sum() / ex[:, 0].sum())
+    tv = lambda a: float(np.abs(np.diff(a)).sum())  # noqa: E731
+    tv_rho, tv_p = tv(rho.mean(axis=1)) / tv(ex[:, 0]), tv(P11.mean(axis=1)) / tv(ex[:, 2])
+    dpt = float(np.abs(P22 / rho - ratio).max())
+    print(f"min P11 {P11.min():.3e}, L1(rho) {l1:.3e}, max|P22/rho - {ratio}| {dpt:.3e}, TV {tv_rho:.4f} {tv_p:.4f}")
+    assert np.isfinite(q).all() and P11.min() > 1e3 and l1 <= 0.05 and dpt < 1e-9
+    assert tv_rho <= 1.05 and tv_p <= 1.05
+
+
+BERTHON_TOL = {"Berthon_SCS": 2.5e-2, "Berthon_RCS": 1.5e-2, "Berthon_RCR": 1.5e-2, "Berthon_SCS-hlle": 2.5e-2}
+BERTHON_Q = {"Density": lambda q: q[0], "X Velocity": lambda q: q[1], "Y Velocity": lambda q: q[2],
+             "XX Pressure": lambda q: q[4], "YY Pressure": lambda q: q[7],
+             "Det Pressure": lambda q: q[4] * q[7] - q[5] * q[5]}
+
+
+def sample_profile(xe, ve, x):
+    """A piecewise-linear profile at x; repeated abscissae are jumps."""
+    if x <= xe[0]:
+        return ve[0]
+    if x >= xe[-1]:
+        return ve[-1]
+    for i in range(len(xe) - 1):
+        if xe[i] <= x <= xe[i + 1] and xe[i + 1] > xe[i]:
+            return ve[i] + (ve[i + 1] - ve[i]) * (x - xe[i]) / (xe[i + 1] - xe[i])
+    return ve[-1]
+
+
+def check_berthon(name, q, d):
+    xc = cell_x(d)
+    worst = 0.0
+    for zone, get in BERTHON_Q.items():
+        xe, ve = d[f"exact_x_{zone}"], d[f"exact_v_{zone}"]
+        ref = np.array([sample_profile(xe, ve, x) for x in xc])
+        num = get(q[:, :, 0])
+        err = float(np.abs(num - ref).sum() / (len(xc) * (ve.max() - ve.min())))
+        print(f"  {zone}: L1 {err:.3e}")
+        worst = max(worst, err)
+    assert np.isfinite(q).all() and worst <= BERTHON_TOL[name]
+
+
 def verify(name, case, ini, d, r, st):
     q = interior(r)
     closure = case.fams[0].closure
+    t_end = float(ini["ICE-Parameters"].get("time-threshold", "0"))
+    assert st.time >= t_end * (1.0 - 1e-12)
     if name in ("Axis_IG", "Axis_AG"):
         check_axis_rest(q, closure)
     elif name == "Axis_MK":
         check_axis_mk(q, d["ic"])
     elif name == "Reflect_AG":
-        assert st.time >= 0.1 * (1.0 - 1e-12)
         check_reflect(q, d["ic"][:, :, :, 0])
+    elif name == "Riemann_AG":
+        check_riemann_ag(q, d, st.time)
+    elif name.startswith("Berthon"):
+        check_berthon(name, q, d)
     else:
-        assert st.time >= 0.1 * (1.0 - 1e-12)
         check_thermal(q, closure)
 
 
-NAMES = ["Axis_IG", "Axis_MK", "Axis_AG", "Reflect_AG", "Thermal_IG", "Thermal_AG"]
+NAMES = ["Axis_IG", "Axis_MK", "Axis_AG", "Reflect_AG", "Thermal_IG", "Thermal_AG", "Riemann_AG", "Berthon_SCS",
+         "Berthon_RCS", "Berthon_RCR", "Berthon_SCS-hlle"]
 
 
 @pytest.mark.parametrize("name", NAMES)
