@@ -120,36 +120,38 @@ def _dir_token(tok: str, where: str) -> Tuple[float, bool]:
         raise FormatError(f"{where}: a direction that is neither a number nor normal: '{tok}'") from None
 
 
-def read_bc(path: str, dims: Sequence[Tuple[int, int, int]], ngroups: int):
-    """Boundary table of an ATLAS bc.txt file with one copy for all families.
-
-    Returns (reci, recd, donor, weight) in the C-ABI layout: reci (nrec,
-    REC_NI) int32 and recd (nrec, REC_ND) float64, nrec = ncell * ngroups in
-    record order idx = (n-1)*ngroups + p; donor (nd, 4) int32 (b, i, j, k) and
-    weight (nd,), record r's donors from reci[r, R_DONOR0] on (layer 1, then 2).
-    """
+def _bc_records(path: str, dims: Sequence[Tuple[int, int, int]]):
+    """Every record of an ATLAS bc.txt in file order (Setup_BC's reading rules,
+    IO_BC.f90:144-326; Count_BC_Records walks the same rules): a list of (ri,
+    rd, donors, weights, line) with ri (REC_NI,) int32 and rd (REC_ND,) float64
+    of the header and payload, and the chimera record's donors and weights."""
     with open(path) as fh:
         lines = fh.read().split("\n")
     pos = 0
 
-    def nxt():
+    def nxt(need=True):
         nonlocal pos
         while pos < len(lines) and not lines[pos].strip():
             pos += 1
         if pos >= len(lines):
-            raise FormatError(f"{path}: the file ends before the last boundary cell")
+            if need:
+                raise FormatError(f"{path}: the file ends before the last boundary cell")
+            return None, None
         pos += 1
         # list-directed input (IO_BC.f90:261-265): commas separate values as blanks do
         return pos, [t for t in re.split(r"[,\s]+", lines[pos - 1].strip()) if t]
 
-    ncell = sum(2 * (nj * nk + ni * nk + ni * nj) for ni, nj, nk in dims)
-    ri = np.zeros((ncell, REC_NI), np.int32)
-    rd = np.zeros((ncell, REC_ND), np.float64)
-    donor: List[Tuple[int, int, int, int]] = []
-    weight: List[float] = []
-    for n in range(ncell):
-        ln, tok = nxt()
+    out = []
+    while True:
+        ln, tok = nxt(need=False)
+        if tok is None:
+            return out
+        n = len(out)
         where = f"{path}, line {ln}, record {n + 1}"
+        ri = np.zeros(REC_NI, np.int32)
+        rd = np.zeros(REC_ND, np.float64)
+        dl: List[Tuple[int, int, int, int]] = []
+        wl: List[float] = []
         if len(tok) < 6:
             raise FormatError(f"{where}: a header needs b i j k f code")
         try:
@@ -165,27 +167,25 @@ def read_bc(path: str, dims: Sequence[Tuple[int, int, int]], ngroups: int):
         on_face = {1: i == 1, 2: i == ni, 3: j == 1, 4: j == nj, 5: k == 1, 6: k == nk}.get(f, False)
         if not (cell_ok and on_face):
             raise FormatError(f"{where}: cell ({b},{i},{j},{k}) is not on face {f} of its block")
-        ri[n, R_B], ri[n, R_I], ri[n, R_J], ri[n, R_K], ri[n, R_F], ri[n, R_TYPE] = b, i, j, k, f, code
+        ri[R_B], ri[R_I], ri[R_J], ri[R_K], ri[R_F], ri[R_TYPE] = b, i, j, k, f, code
         if code in (101, 201):
             ln, tok = nxt()
             if len(tok) < 9:
                 raise FormatError(f"{path}, line {ln}: a connection payload needs bs is js ks fs d11 d12 d21 d22")
-            ri[n, R_BS], ri[n, R_IS], ri[n, R_JS], ri[n, R_KS], ri[n, R_FS] = (int(x) for x in tok[:5])
+            ri[R_BS], ri[R_IS], ri[R_JS], ri[R_KS], ri[R_FS] = (int(x) for x in tok[:5])
         elif code == 102:
             ln, tok = nxt()
             n1, n2 = int(tok[0]), int(tok[1])
-            ri[n, R_NCHI1], ri[n, R_NCHI2], ri[n, R_DONOR0] = n1, n2, len(donor)
-            w = []
+            ri[R_NCHI1], ri[R_NCHI2] = n1, n2
             for _ in range(n1 + n2):
                 ln, tok = nxt()
                 db, di, dj, dk = (int(x) for x in tok[:4])
                 if not 1 <= db <= len(dims):
                     raise FormatError(f"{path}, line {ln}: donor block {db} is not in the mesh")
-                donor.append((db, di, dj, dk))
-                w.append(float(tok[4]))
-            if sum(w[:n1]) < 0.5 or sum(w[n1:]) < 0.5:
+                dl.append((db, di, dj, dk))
+                wl.append(float(tok[4]))
+            if sum(wl[:n1]) < 0.5 or sum(wl[n1:]) < 0.5:
                 raise FormatError(f"{where}: a chimera ghost layer whose donor weights sum to ~0")
-            weight.extend(w)
         elif 401 <= code <= 403:
             ln, tok = nxt()
             if len(tok) < 6:
@@ -194,21 +194,91 @@ def read_bc(path: str, dims: Sequence[Tuple[int, int, int]], ngroups: int):
             be, nb = _dir_token(tok[3], f"{path}, line {ln}")
             if na != nb:
                 raise FormatError(f"{path}, line {ln}: alpha and beta are both normal or both numbers")
-            rd[n, RD_MASSFLUX], rd[n, RD_VELOCITY] = float(tok[0]), float(tok[1])
-            rd[n, RD_ALPHA], rd[n, RD_BETA] = a, be
-            rd[n, RD_TEMPERATURE], rd[n, RD_RADIUS] = float(tok[4]), float(tok[5])
-            ri[n, R_NORMAL_DIR] = int(na)
+            rd[RD_MASSFLUX], rd[RD_VELOCITY] = float(tok[0]), float(tok[1])
+            rd[RD_ALPHA], rd[RD_BETA] = a, be
+            rd[RD_TEMPERATURE], rd[RD_RADIUS] = float(tok[4]), float(tok[5])
+            ri[R_NORMAL_DIR] = int(na)
             if code == 401:
                 if len(tok) < 10:
                     raise FormatError(f"{path}, line {ln}: a 401 line without its 10th field G")
-                rd[n, RD_GFIXED] = float(tok[9])
-            if rd[n, RD_MASSFLUX] != 0.0 and (rd[n, RD_VELOCITY] == 0.0 or not rd[n, RD_RADIUS] > 0.0):
+                rd[RD_GFIXED] = float(tok[9])
+            if rd[RD_MASSFLUX] != 0.0 and (rd[RD_VELOCITY] == 0.0 or not rd[RD_RADIUS] > 0.0):
                 raise FormatError(f"{path}, line {ln}: a mass flux with a zero velocity or a non-positive radius")
         elif code not in (0, 200, 300, 301, 400):
             raise FormatError(f"{where}: unsupported ATLAS BC code {code}")
-    reci = np.repeat(ri, ngroups, axis=0)
-    recd = np.repeat(rd, ngroups, axis=0)
-    reci[:, R_P] = np.tile(np.arange(1, ngroups + 1, dtype=np.int32), ncell)
+        out.append((ri, rd, dl, wl, ln))
+
+
+def read_bc(path: str, dims: Sequence[Tuple[int, int, int]], ngroups: int):
+    """Boundary table of an ATLAS bc.txt file (Setup_BC, IO_BC.f90:65-344).
+
+    The file holds one copy of the table for all families, or one copy per
+    family (one per material and population pair), block by block with every
+    copy of a block together over the same cells; copy c then belongs to family
+    c.  Any other number of copies is refused.  Returns (reci, recd, donor,
+    weight) in the C-ABI layout: reci (nrec, REC_NI) int32 and recd (nrec,
+    REC_ND) float64, nrec = ncell * ngroups in record order idx = (n-1)*ngroups
+    + p; donor (nd, 4) int32 (b, i, j, k) and weight (nd,), record r's donors
+    from reci[r, R_DONOR0] on (layer 1, then 2).
+    """
+    recs = _bc_records(path, dims)
+    per_block = [2 * (nj * nk + ni * nk + ni * nj) for ni, nj, nk in dims]
+    ncell = sum(per_block)
+    if len(recs) < ncell:
+        raise FormatError(f"{path}: the file ends before the last boundary cell")
+    if len(recs) % ncell:
+        raise FormatError(f"{path}: the file holds {len(recs)} records, which is not a whole number of copies "
+                          f"of the {ncell} boundary cells of this mesh")
+    ncopy = len(recs) // ncell
+    if ncopy not in (1, ngroups):
+        raise FormatError(f"{path}: the file carries {ncopy} copies of the boundary table but this run has "
+                          f"{ngroups} particle families")
+    reci = np.zeros((ncell * ngroups, REC_NI), np.int32)
+    recd = np.zeros((ncell * ngroups, REC_ND), np.float64)
+    donor: List[Tuple[int, int, int, int]] = []
+    weight: List[float] = []
+
+    def put(n, p, rec, shared_donor=None):
+        ri, rd, dl, wl, _ = rec
+        idx = n * ngroups + (p - 1)
+        reci[idx], recd[idx] = ri, rd
+        reci[idx, R_P] = p
+        if ri[R_TYPE] == 102:
+            if shared_donor is None:
+                reci[idx, R_DONOR0] = len(donor)
+                donor.extend(dl)
+                weight.extend(wl)
+            else:
+                reci[idx, R_DONOR0] = shared_donor
+        return reci[idx, R_DONOR0]
+
+    if ncopy == 1:
+        # one record for every family, each carrying its own (b, i, j, k, f): any order
+        for n, rec in enumerate(recs):
+            d0 = put(n, 1, rec)
+            for p in range(2, ngroups + 1):
+                put(n, p, rec, shared_donor=d0)
+    else:
+        pos, base = 0, 0
+        for b, nbb in enumerate(per_block, 1):
+            keys = []
+            for c in range(1, ncopy + 1):
+                for r in range(nbb):
+                    rec = recs[pos]
+                    pos += 1
+                    ri, ln = rec[0], rec[4]
+                    if ri[R_B] != b:
+                        raise FormatError(f"{path}, line {ln}: record {pos} belongs to block {ri[R_B]} where the mesh "
+                                          f"expects block {b}: a multi-population file lists the blocks in mesh "
+                                          f"order, every copy of one block together")
+                    key = (int(ri[R_I]), int(ri[R_J]), int(ri[R_K]), int(ri[R_F]))
+                    if c == 1:
+                        keys.append(key)
+                    elif keys[r] != key:
+                        raise FormatError(f"{path}, line {ln}: copy {c} of block {b} does not repeat the cells of "
+                                          f"copy 1 in the same order")
+                    put(base + r, c, rec)
+            base += nbb
     dn = np.asarray(donor, np.int32).reshape(-1, 4)
     return (np.ascontiguousarray(reci), np.ascontiguousarray(recd), np.ascontiguousarray(dn),
             np.asarray(weight, np.float64))
@@ -236,11 +306,20 @@ def read_ini(path: str) -> Dict[str, Dict[str, str]]:
 
 
 def write_bc(path: str, reci: np.ndarray, recd: np.ndarray, donor: np.ndarray, weight: np.ndarray,
-             ngroups: int = 1) -> None:
-    """An ATLAS bc.txt with one copy for all families from a boundary table in the
-    C-ABI layout (the inverse of read_bc for family 1's records)."""
+             ngroups: int = 1, copies: bool = False) -> None:
+    """An ATLAS bc.txt from a boundary table in the C-ABI layout: one copy for all
+    families (family 1's records), or with copies one copy per family, block by
+    block, as ATLAS writes a multi-population phase (the inverse of read_bc)."""
+    order = list(range(0, reci.shape[0], ngroups))
+    if copies and ngroups > 1:
+        order = []
+        blocks = reci[::ngroups, R_B]
+        for b in range(1, int(blocks.max()) + 1):
+            cells = [n for n in range(0, reci.shape[0], ngroups) if reci[n, R_B] == b]
+            for p in range(ngroups):
+                order.extend(n + p for n in cells)
     with open(path, "w") as fh:
-        for n in range(0, reci.shape[0], ngroups):
+        for n in order:
             r, d = reci[n], recd[n]
             code = int(r[R_TYPE])
             fh.write(f"{r[R_B]:8d}{r[R_I]:8d}{r[R_J]:8d}{r[R_K]:8d}{r[R_F]:8d}{code:8d}\n")

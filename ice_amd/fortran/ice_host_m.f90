@@ -584,9 +584,11 @@ contains
   end subroutine
 
   ! ------------------------------------------------------------------ BC
-  !> The ATLAS boundary file with one copy for all families (Setup_BC,
-  !> IO_BC.f90:144-326), into the C-ABI's record arrays: reci(ICE_REC_NI, nrec),
-  !> recd(ICE_REC_ND, nrec), record idx = (n-1)*ngroups + p.
+  !> The ATLAS boundary file (Setup_BC, IO_BC.f90:65-344) into the C-ABI's record
+  !> arrays: reci(ICE_REC_NI, nrec), recd(ICE_REC_ND, nrec), record idx =
+  !> (n-1)*ngroups + p.  The file holds one copy of the table for all families, or
+  !> one copy per family, block by block with every copy of a block together over
+  !> the same cells (copy c is family c); any other number of copies is refused.
   subroutine read_bc(path, blocks, ngroups, reci, recd, donor, weight)
     character(len=*), intent(in) :: path
     type(host_block_t), intent(in) :: blocks(:)
@@ -594,6 +596,8 @@ contains
     integer(I4), allocatable, intent(out) :: reci(:,:), donor(:,:)
     real(R8), allocatable, intent(out) :: recd(:,:), weight(:)
     integer :: u, ios, nline, ncell, n, b, i, j, k, f, code, p, s, nchi(2), ndon, cs(9)
+    integer :: nrec, ncopy, c, r, pos, base, nbb
+    logical :: eof
     integer, allocatable :: ri(:,:), dtmp(:,:)
     real(R8), allocatable :: rd(:,:), wtmp(:)
     character(len=1024) :: line
@@ -605,15 +609,22 @@ contains
       ncell = ncell + 2 * (blocks(b)%dim(2) * blocks(b)%dim(3) + blocks(b)%dim(1) * blocks(b)%dim(3) + &
                            blocks(b)%dim(1) * blocks(b)%dim(2))
     end do
-    allocate(ri(0:ICE_REC_NI-1, ncell), rd(0:ICE_REC_ND-1, ncell), dtmp(4, 1024), wtmp(1024))
+    allocate(ri(0:ICE_REC_NI-1, ncell * max(ngroups, 1)), rd(0:ICE_REC_ND-1, ncell * max(ngroups, 1)), &
+             dtmp(4, 1024), wtmp(1024))
     ri = 0
     rd = 0
     ndon = 0
     nline = 0
     open(newunit=u, file=path, status='old', action='read', iostat=ios)
     if (ios /= 0) call refuse('Setup_BC', 'cannot open '//trim(path))
-    do n = 1, ncell
-      call next_line()
+    ! every record of the file, in file order (Count_BC_Records walks the same rules)
+    nrec = 0
+    do
+      call next_line(eof)
+      if (eof) exit
+      n = nrec + 1
+      if (n > size(ri, 2)) call line_error('more records than one copy of the boundary table per family')
+      nrec = n
       read(line, *, iostat=ios) b, i, j, k, f, code
       if (ios /= 0) call line_error('a header needs b i j k f code')
       if (code /= 0 .and. (code < 100 .or. code > 999)) call line_error('bad BC code '//itoa(code))
@@ -675,14 +686,49 @@ contains
       end select
     end do
     close(u)
+    if (nrec < ncell) call refuse('Setup_BC', trim(path)//': the file ends before the last boundary cell')
+    if (mod(nrec, ncell) /= 0) call refuse('Setup_BC', trim(path)//': the file holds '//itoa(nrec)// &
+      ' records, which is not a whole number of copies of the '//itoa(ncell)//' boundary cells of this mesh')
+    ncopy = nrec / ncell
+    if (ncopy /= 1 .and. ncopy /= ngroups) call refuse('Setup_BC', trim(path)//': the file carries '//itoa(ncopy)// &
+      ' copies of the boundary table but this run has '//itoa(ngroups)//' particle families')
     allocate(reci(0:ICE_REC_NI-1, ncell * ngroups), recd(0:ICE_REC_ND-1, ncell * ngroups))
-    do n = 1, ncell
-      do p = 1, ngroups
-        reci(:, (n - 1) * ngroups + p) = ri(:, n)
-        reci(R_P, (n - 1) * ngroups + p) = p
-        recd(:, (n - 1) * ngroups + p) = rd(:, n)
+    if (ncopy == 1) then
+      ! one record for every family, each carrying its own cell: any order
+      do n = 1, ncell
+        do p = 1, ngroups
+          reci(:, (n - 1) * ngroups + p) = ri(:, n)
+          reci(R_P, (n - 1) * ngroups + p) = p
+          recd(:, (n - 1) * ngroups + p) = rd(:, n)
+        end do
       end do
-    end do
+    else
+      ! block by block, the copies of a block together, each over copy 1's cells
+      pos = 0
+      base = 0
+      do b = 1, size(blocks)
+        nbb = 2 * (blocks(b)%dim(2) * blocks(b)%dim(3) + blocks(b)%dim(1) * blocks(b)%dim(3) + &
+                   blocks(b)%dim(1) * blocks(b)%dim(2))
+        do c = 1, ncopy
+          do r = 1, nbb
+            pos = pos + 1
+            if (ri(R_B, pos) /= b) call refuse('Setup_BC', trim(path)//': record '//itoa(pos)//' belongs to block '// &
+              itoa(ri(R_B, pos))//' where the mesh expects block '//itoa(b)//': a multi-population file lists '// &
+              'the blocks in mesh order, every copy of one block together')
+            if (c > 1) then
+              if (any(ri([R_I, R_J, R_K, R_F], pos) /= ri([R_I, R_J, R_K, R_F], pos - (c - 1) * nbb))) &
+                call refuse('Setup_BC', trim(path)//': copy '//itoa(c)//' of block '//itoa(b)// &
+                            ' does not repeat the cells of copy 1 in the same order')
+            end if
+            n = base + r
+            reci(:, (n - 1) * ngroups + c) = ri(:, pos)
+            reci(R_P, (n - 1) * ngroups + c) = c
+            recd(:, (n - 1) * ngroups + c) = rd(:, pos)
+          end do
+        end do
+        base = base + nbb
+      end do
+    end if
     allocate(donor(4, max(ndon, 1)), weight(max(ndon, 1)))
     donor = 0
     weight = 0
@@ -691,10 +737,20 @@ contains
 
   contains
 
-    subroutine next_line()
+    !> the next non-blank line; eof (when present) says that the file ended there,
+    !> else the end of the file stops the setup
+    subroutine next_line(eof)
+      logical, intent(out), optional :: eof
+      if (present(eof)) eof = .false.
       do
         read(u, '(A)', iostat=ios) line
-        if (ios /= 0) call refuse('Setup_BC', trim(path)//': the file ends before the last boundary cell')
+        if (ios /= 0) then
+          if (present(eof)) then
+            eof = .true.
+            return
+          end if
+          call refuse('Setup_BC', trim(path)//': the file ends before the last boundary cell')
+        end if
         nline = nline + 1
         if (len_trim(line) > 0) exit
       end do

@@ -34,8 +34,10 @@ program ice_driver
   type(c_ptr) :: ctx
   integer :: device, nb, b, nc, ndir, iters, it, n, shell, rc, ndonor, mgl, m, total_iters
   integer(I8) :: cells, t0, t1, rate
-  character(len=96) :: names
+  character(len=2048) :: names
   character(len=16) :: key
+  integer :: p, nvar
+  integer :: off(14), ncf(14)
 
   call get_command_argument(1, dir)
   if (len_trim(dir) == 0) call refuse('ice_driver', 'usage: ice_driver <case directory> [device]')
@@ -44,12 +46,20 @@ program ice_driver
   if (len_trim(arg) > 0) read(arg, *) device
 
   call read_registry(trim(dir)//'/input.ini', reg)
-  if (reg%nfam /= 1) call refuse('ice_driver', 'the initial condition file holds one family')
-  select case (trim(reg%closure(1)))
-  case ('MK'); nc = 6;  names = 'rho_p1 u_p1 v_p1 w_p1 T_p1 n_p1'
-  case ('IG'); nc = 7;  names = 'rho_p1 u_p1 v_p1 w_p1 P_p1 T_p1 n_p1'
-  case default; nc = 12; names = 'rho_p1 u_p1 v_p1 w_p1 P11_p1 P12_p1 P13_p1 P22_p1 P23_p1 P33_p1 T_p1 n_p1'
-  end select
+  ! the initial condition holds every family's primitive variables, family after
+  ! family ([ICE-Family<p>] closure), as ICE's part-ic.tec does
+  nvar = 0
+  names = ''
+  do p = 1, reg%nfam
+    select case (trim(reg%closure(p)))
+    case ('MK'); ncf(p) = 6;  call add_names('rho u v w T n', p)
+    case ('IG'); ncf(p) = 7;  call add_names('rho u v w P T n', p)
+    case default; ncf(p) = 12; call add_names('rho u v w P11 P12 P13 P22 P23 P33 T n', p)
+    end select
+    off(p) = nvar
+    nvar = nvar + ncf(p)
+  end do
+  nc = nvar
   mgl = reg_int(reg, 'levels')
   if (mgl < 1 .or. mgl > MAXLEV) call refuse('Register_Multigrid_Levels', 'levels must be 1 to 4 here')
   call read_ic(trim(dir)//'/INPUT/part-ic.tec', nc, blocks)
@@ -108,7 +118,9 @@ program ice_driver
     end if
     if (m == 1) then
       do b = 1, nb
-        call check(ice_gpu_put_state(ctx, b, 1, lev(1)%b(b)%prim), 'put_state')
+        do p = 1, cfg%ngroups
+          call check(ice_gpu_put_state(ctx, b, p, lev(1)%b(b)%prim(off(p) + 1:off(p) + ncf(p), :, :, :)), 'put_state')
+        end do
       end do
       ! res-threshold ends steady runs, on the finest level (Mod_Explicit.f90:196-200)
       call check(ice_gpu_set_res_threshold(ctx, reg_real(reg, 'res-threshold')), 'set_res_threshold')
@@ -190,7 +202,9 @@ program ice_driver
 
   ctx = lev(1)%ctx
   do b = 1, nb
-    call check(ice_gpu_get_state(ctx, b, 1, lev(1)%b(b)%prim), 'get_state')
+    do p = 1, cfg%ngroups
+      call check(ice_gpu_get_state(ctx, b, p, lev(1)%b(b)%prim(off(p) + 1:off(p) + ncf(p), :, :, :)), 'get_state')
+    end do
   end do
   call execute_command_line('mkdir -p '//trim(dir)//'/OUTPUT')
   call write_solution(trim(dir)//'/OUTPUT/part-field.tec', names, lev(1)%b, st%iter)
@@ -214,6 +228,19 @@ contains
     gK = gas(b)%prim(8, 1:d(1), 1:d(2), 1:d(3))
     gM = gas(b)%prim(9, 1:d(1), 1:d(2), 1:d(3))
     call check(ice_gpu_put_gas(ctx, b, g5, gR, gG, gK, gM), 'put_gas')
+  end subroutine
+
+  !> the names of family p's variables, suffixed _p<p>
+  subroutine add_names(words, p)
+    character(len=*), intent(in) :: words
+    integer, intent(in) :: p
+    character(len=8) :: w(12)
+    integer :: q, ios
+    w = ''
+    read(words, *, iostat=ios) w
+    do q = 1, 12
+      if (len_trim(w(q)) > 0) names = trim(names)//' '//trim(w(q))//'_p'//trim(itoa_m(p))
+    end do
   end subroutine
 
   function itoa_m(i) result(s)

@@ -250,3 +250,91 @@ def test_driver_coupled_on_gpu(tmp_path, oracle, which, mode):
     assert p.returncode == 0
     for g, r in zip(field_of(d), oracle_reference(oracle, case)):
         assert rel_l2(g, r) <= 1e-12
+
+
+def write_populations(d, case, copies, mode="plain"):
+    """The three-family duct of test_populations in ICE's input files: input.ini with
+    [ICE-Family1..3], part-ic.tec with the families' variables one after the other,
+    part-bc.txt with one copy per family (or one for all) and a constant property
+    table."""
+    os.makedirs(os.path.join(d, "INPUT"), exist_ok=True)
+    cfg, fam = case.cfg, case.fams[0]
+    ini = {"ICE-Parameters": {"iter-threshold": str(case.steps)},
+           "ICE-Numerics": {"time-scheme": "RK2", "cfl": repr(cfg.cfl), "dt-max": repr(cfg.dt_max),
+                            "time-accurate": ".true.", "space-reconstruction": "MUSCL", "flux-limiter": "vanleer"},
+           "ICE-IO": {"shell-diter": str(case.steps)}, "ICE-Device": {"mode": mode}}
+    for p in range(1, len(case.fams) + 1):
+        ini[f"ICE-Family{p}"] = {"closure": "MK"}
+    formats.write_ini(os.path.join(d, "input.ini"), ini)
+    B = case.blocks[0]
+    ni, nj, nk = B.dims
+    q = np.concatenate([pr.reshape((6, ni + 4, nj + 4, nk + 4), order="F")[:, 2:ni + 2, 2:nj + 2, 2:nk + 2]
+                        for pr in B.prim])
+    from ice_amd import geometry
+    nodes = geometry.cartesian_nodes(np.linspace(0.0, 0.16, ni + 1), np.linspace(0.0, 0.04, nj + 1), [0.0, 0.01])
+    formats.write_tecplot(os.path.join(d, "INPUT", "part-ic.tec"), ["x", "y", "z"] + [f"v{m}" for m in range(q.shape[0])],
+                          [formats.Zone("B", nodes.shape[1:], nodes, np.ascontiguousarray(q), {})])
+    formats.write_bc(os.path.join(d, "INPUT", "part-bc.txt"), case.reci, case.recd, case.donor, case.weight,
+                     ngroups=len(case.fams), copies=copies > 1)
+    with open(os.path.join(d, "INPUT", "part-properties.dat"), "w") as fh:
+        fh.write('TITLE = "Mass Thermodynamic Properties"\nVARIABLES = "Temperature", "Cp", "Density", "Enthalpy"\n')
+        fh.write('ZONE T="A"\nI=3, F=POINT\n')
+        for T in (1.0, 2.0, 3.0):
+            fh.write(f"{T} {fam.cs!r} {fam.rho_mat!r} {fam.cs * T!r}\n")
+
+
+def populations_reference(oracle, case):
+    from ice_amd import cases
+    r = cases.Runner(oracle, case, "orc_")
+    r.step(case.steps)
+    out = np.concatenate([r.state(1, p) for p in range(1, len(case.fams) + 1)])
+    r.close()
+    return out
+
+
+@pytest.mark.parametrize("copies", [1, 3])
+def test_driver_families_on_oracle_abi(tmp_path, oracle, copies):
+    """Three families through the Fortran host, their boundary table in one copy or
+    one copy per family (ICE's test/fast/populations): the same states as the oracle
+    run from Python, bit for bit, and with copies family p carries 2^(p-1) times
+    family 1's loading."""
+    from test_populations import duct
+    case = duct(str(tmp_path), copies, 3, scaled=copies > 1)
+    d = str(tmp_path / "case")
+    write_populations(d, case, copies)
+    p = run_driver(d, {"LD_LIBRARY_PATH": ABI})
+    print(p.stdout[-600:], p.stderr[-600:])
+    assert p.returncode == 0
+    got = field_of(d)[0]
+    assert got.shape[0] == 18
+    assert np.array_equal(got, populations_reference(oracle, case))
+    if copies > 1:
+        assert np.allclose(got[12], 4.0 * got[0], rtol=1e-12, atol=0)
+
+
+def test_driver_refuses_copy_mismatch(tmp_path):
+    from test_populations import duct
+    d = str(tmp_path / "case")
+    write_populations(d, duct(str(tmp_path), 1, 3, scaled=False), 1)
+    # two copies for the three families
+    two = duct(str(tmp_path), 2, 2, scaled=False)
+    formats.write_bc(os.path.join(d, "INPUT", "part-bc.txt"), two.reci, two.recd, two.donor, two.weight,
+                     ngroups=2, copies=True)
+    p = run_driver(d, {"LD_LIBRARY_PATH": ABI})
+    assert p.returncode != 0 and "copies of the boundary table" in p.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_driver_families_on_gpu(tmp_path, oracle, mode):
+    """The three-family duct with one boundary copy per family through the Fortran
+    host on the GPU: the oracle's states bit for bit."""
+    print(f"case fortran driver families {mode}", flush=True)
+    from test_populations import duct
+    case = duct(str(tmp_path), 3, 3, scaled=True)
+    d = str(tmp_path / "case")
+    write_populations(d, case, 3, mode)
+    p = run_driver(d, {}, timeout=300)
+    print(p.stdout[-400:], p.stderr[-400:])
+    assert p.returncode == 0
+    assert np.array_equal(field_of(d)[0], populations_reference(oracle, case))
