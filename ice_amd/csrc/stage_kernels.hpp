@@ -941,24 +941,38 @@ ICE_DEV void cell_slopes(const FamArgs& a, long long g, long long s, int d, int 
   int lim = a.limiter;
   if constexpr (LIM >= 0) lim = LIM;
   const long long N = a.N;
+  // every load of the cell (its three cells' slots, lengths and inflow flags) is
+  // issued before the first-order test, so they make one round trip to memory
+  // instead of two; the empty asm keeps the compiler from moving them below the
+  // test's branch
+  const double* p = a.cur;
+  double prev[NC], local[NC], next[NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    const double* pv = p + (long long)v * N;
+    prev[v] = pv[g - s];
+    local[v] = pv[g];
+    next[v] = pv[g + s];
+  }
   const double* len = a.dl + (long long)d * N;
   double len0 = len[g - s], len1 = len[g], len2 = len[g + s];
-  if (m == 1 && a.inj[g - s]) len0 = 0.0;
-  if (m == n && a.inj[g + s]) len2 = 0.0;
+  const bool inj0 = a.inj[g - s], inj2 = a.inj[g + s];
+  if (m == 1 && inj0) len0 = 0.0;
+  if (m == n && inj2) len2 = 0.0;
   const double dl0 = 0.5 * (len0 + len1), dl1 = 0.5 * (len1 + len2);
   half = 0.5 * len1;
-  const double* p = a.cur;
-  first = p[g - s] <= kRhoEmpty || p[g] <= kRhoEmpty || p[g + s] <= kRhoEmpty;
+#pragma unroll
+  for (int v = 0; v < NC; ++v) asm volatile("" ::"v"(prev[v]), "v"(local[v]), "v"(next[v]));
+  asm volatile("" ::"v"(dl0), "v"(dl1));
+  first = prev[0] <= kRhoEmpty || local[0] <= kRhoEmpty || next[0] <= kRhoEmpty;
   if (first) return;
   double cx, cy;
   if (half > 0.0) { cx = dl1 / half; cy = dl0 / half; } else { cx = 2.0; cy = 2.0; }
   const SharedDiv by0 = div_prepare(dl0), by1 = div_prepare(dl1);
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
-    const double* pv = p + (long long)v * N;
-    const double prev = pv[g - s], local = pv[g], next = pv[g + s];
-    const double s_up = div_shared(next - local, by1);
-    const double s_dn = div_shared(local - prev, by0);
+    const double s_up = div_shared(next[v] - local[v], by1);
+    const double s_dn = div_shared(local[v] - prev[v], by0);
     slope[v * kMaxExtCells + e] = limiter(lim, s_up, s_dn, cx, cy);
   }
 }
@@ -973,22 +987,31 @@ ICE_DEV bool face_from_slopes(const FamArgs& a, long long g, long long s, int d,
                               const double* half, const uint8_t* first, int elo, int ehi, double* q) {
   constexpr int NC = Closure<M>::NC;
   const long long N = a.N;
-  const double* p = a.cur;
+  // the face's geometry and both cells' values are requested first, in one round
+  // trip to memory, ahead of the LDS reads and the branches that use them
+  const double* fa = a.fa + (long long)(d * 4) * N;
+  const double nrm[3] = {fa[g], fa[N + g], fa[2 * N + g]};
+  const double area = fa[3 * N + g];
+  double lo[NC], hi[NC];
+  load_cell<NC>(a.cur, N, g, lo);
+  load_cell<NC>(a.cur, N, g + s, hi);
   const bool first_l = first[elo], first_r = first[ehi];
   const double dll = half[elo], dlr = half[ehi];
   const double bf = SD ? dmin2(a.beta[g], a.beta[g + s]) : 1.0;
   double ql[NC], qr[NC];
   bool ok = true;
   if (first_l && first_r) {
-    load_cell<NC>(p, N, g, ql);
-    load_cell<NC>(p, N, g + s, qr);
+#pragma unroll
+    for (int v = 0; v < NC; ++v) {
+      ql[v] = lo[v];
+      qr[v] = hi[v];
+    }
   } else {
     double limval = bf;
 #pragma unroll
     for (int v = 0; v < NC; ++v) {
-      const double lo = p[(long long)v * N + g], hi = p[(long long)v * N + g + s];
-      ql[v] = first_l ? lo : lo + slope[v * kMaxExtCells + elo] * limval * dll;
-      qr[v] = first_r ? hi : hi - slope[v * kMaxExtCells + ehi] * limval * dlr;
+      ql[v] = first_l ? lo[v] : lo[v] + slope[v * kMaxExtCells + elo] * limval * dll;
+      qr[v] = first_r ? hi[v] : hi[v] - slope[v * kMaxExtCells + ehi] * limval * dlr;
     }
     bool ok_l = first_l || checkprim<M>(ql), ok_r = first_r || checkprim<M>(qr);
     if (__builtin_expect(!(ok_l && ok_r), 0)) {
@@ -996,16 +1019,18 @@ ICE_DEV bool face_from_slopes(const FamArgs& a, long long g, long long s, int d,
       for (;;) {
         limval = 0.5 * limval;
         if (limval < 1.e-18) {
-          load_cell<NC>(p, N, g, ql);
-          load_cell<NC>(p, N, g + s, qr);
+#pragma unroll
+          for (int v = 0; v < NC; ++v) {
+            ql[v] = lo[v];
+            qr[v] = hi[v];
+          }
           ok = checkprim<M>(ql) && checkprim<M>(qr);
           break;
         }
 #pragma unroll
         for (int v = 0; v < NC; ++v) {
-          const double lo = p[(long long)v * N + g], hi = p[(long long)v * N + g + s];
-          if (!first_l) ql[v] = lo + slope[v * kMaxExtCells + elo] * limval * dll;
-          if (!first_r) qr[v] = hi - slope[v * kMaxExtCells + ehi] * limval * dlr;
+          if (!first_l) ql[v] = lo[v] + slope[v * kMaxExtCells + elo] * limval * dll;
+          if (!first_r) qr[v] = hi[v] - slope[v * kMaxExtCells + ehi] * limval * dlr;
         }
         if (!first_l) ok_l = checkprim<M>(ql);
         if (!first_r) ok_r = checkprim<M>(qr);
@@ -1013,9 +1038,7 @@ ICE_DEV bool face_from_slopes(const FamArgs& a, long long g, long long s, int d,
       }
     }
   }
-  const double* fa = a.fa + (long long)(d * 4) * N;
-  const double nrm[3] = {fa[g], fa[N + g], fa[2 * N + g]};
-  riemann_acc<M, S, 2>(ql, qr, nrm, fa[3 * N + g], q, mat_of<TAB>(a));
+  riemann_acc<M, S, 2>(ql, qr, nrm, area, q, mat_of<TAB>(a));
   return ok;
 }
 
