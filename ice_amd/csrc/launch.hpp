@@ -44,24 +44,51 @@ inline unsigned grid_cells(long long n) { return (unsigned)((n + kThreads - 1) /
 
 // Implicit residual smoothing of a family over every block: one launch per direction,
 // both Jacobi sweeps fused, from R into T, then T into R, and so on: along i one
-// thread per cell, along j and k one per line and slot, marching.  Returns the
-// buffer that holds the smoothed residual.
+// thread per cell, along j and k one per line and slot, marching; with a direction
+// j the march along j also forms the smoothing along i (k_irs_march FOLD_I), so
+// direction i has no launch of its own.  Returns the buffer that holds the
+// smoothed residual.
 template <int NC>
 const double* launch_irs(const FamArgs& a, int ndir, const int* wg, int nwg, const int* const* wl, const int* nwl,
                          int nb, hipStream_t stream) {
   const double* src = a.R;
   double* dst = a.T;
   for (int d = 0; d < ndir; ++d) {
+    if (d == 0 && ndir > 1) continue;  // folded into the march along j
     if (d == 0)
       hipLaunchKernelGGL(k_irs<NC>, dim3((unsigned)nwg), dim3(kThreads), 0, stream, a, d, src, dst, wg, nb);
+    else if (d == 1)
+      hipLaunchKernelGGL((k_irs_march<NC, true>), dim3((unsigned)nwl[0], NC), dim3(kThreads), 0, stream, a, d, src,
+                         dst, wl[0], nb);
     else
-      hipLaunchKernelGGL(k_irs_march<NC>, dim3((unsigned)nwl[d - 1], NC), dim3(kThreads), 0, stream, a, d, src, dst,
-                         wl[d - 1], nb);
+      hipLaunchKernelGGL((k_irs_march<NC, false>), dim3((unsigned)nwl[d - 1], NC), dim3(kThreads), 0, stream, a, d,
+                         src, dst, wl[d - 1], nb);
     double* t = const_cast<double*>(src);
     src = dst;
     dst = t;
   }
   return src;
+}
+
+// The smoothing followed by the RK update (k_update): with two or three
+// directions the last direction's march does the update itself
+// (k_irs_march_update), and the smoothing along i is formed inside the march along
+// j.  R holds the residual on entry.
+template <int M>
+void launch_irs_update(const FamArgs& a, int ndir, const int* wg, int nwg, const int* const* wl, const int* nwl, int nb,
+                       hipStream_t stream) {
+  constexpr int NC = Closure<M>::NC;
+  if (ndir < 2) {
+    const double* res = launch_irs<NC>(a, ndir, wg, nwg, wl, nwl, nb, stream);
+    hipLaunchKernelGGL(k_update<M>, dim3((unsigned)nwg), dim3(kThreads), 0, stream, a, res, wg, nb);
+  } else if (ndir == 2) {
+    hipLaunchKernelGGL((k_irs_march_update<M, true>), dim3((unsigned)nwl[0]), dim3(kThreads), 0, stream, a, 1, a.R,
+                       wl[0], nb);
+  } else {
+    const double* res = launch_irs<NC>(a, 2, wg, nwg, wl, nwl, nb, stream);
+    hipLaunchKernelGGL((k_irs_march_update<M, false>), dim3((unsigned)nwl[1]), dim3(kThreads), 0, stream, a, 2, res,
+                       wl[1], nb);
+  }
 }
 
 template <int M, int S, bool SD, int CP>
@@ -139,10 +166,7 @@ void launch_stage_kernels(const StageLaunch& L, FamArgs a) {
       L.lkev->push_back(e1);
     }
   }
-  if (L.irs) {
-    const double* res = launch_irs<NC>(a, L.ndir, L.irs_wg, L.irs_nwg, L.irs_wl, L.irs_nwl, nb, L.stream);
-    hipLaunchKernelGGL(k_update<M>, dim3((unsigned)L.irs_nwg), dim3(kThreads), 0, L.stream, a, res, L.irs_wg, nb);
-  }
+  if (L.irs) launch_irs_update<M>(a, L.ndir, L.irs_wg, L.irs_nwg, L.irs_wl, L.irs_nwl, nb, L.stream);
 }
 
 // One closure, Riemann solver and sensor setting's stage kernels: instantiated in
@@ -262,12 +286,8 @@ template <int M, int S, bool SD, int CP> void launch_batched_stage(const BatchLa
     if (B.ev1) (void)hipEventRecord(B.ev1, B.stream);
     finish_lk_group<M, CP>(B, true);
     if (B.ev2) (void)hipEventRecord(B.ev2, B.stream);
-    constexpr int NC = Closure<M>::NC;
-    for (int f = 0; f < B.nfam; ++f) {
-      FamArgs a = B.h_fams[B.fam_ids[f]];
-      const double* res = launch_irs<NC>(a, B.ndir, B.irs_wg, B.irs_nwg, B.irs_wl, B.irs_nwl, B.nb, B.stream);
-      hipLaunchKernelGGL(k_update<M>, dim3((unsigned)B.irs_nwg), blk, 0, B.stream, a, res, B.irs_wg, B.nb);
-    }
+    for (int f = 0; f < B.nfam; ++f)
+      launch_irs_update<M>(B.h_fams[B.fam_ids[f]], B.ndir, B.irs_wg, B.irs_nwg, B.irs_wl, B.irs_nwl, B.nb, B.stream);
     return;
   }
   if (B.dispatch != 0) {

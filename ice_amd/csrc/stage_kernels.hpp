@@ -805,12 +805,28 @@ __global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int d, const double
   for (int v = 0; v < NC; ++v) dst[(long long)v * a.N + g] = r[v];
 }
 
+// The value the smoothing along i leaves at cell g (index i of ni along i) for one
+// slot Rv: irs_line's expressions for that slot, bit for bit.  g-2 .. g+2 are all
+// inside the block's padded box, so the loads are unconditional and the ends select.
+ICE_DEV double irs_i_value(const double* __restrict__ Rv, long long g, int i, int ni, double beta, double den) {
+  const double x0 = Rv[g - 2], x1 = Rv[g - 1], x2 = Rv[g], x3 = Rv[g + 1], x4 = Rv[g + 2];
+  const double rmm = i - 1 > 1 ? x0 : x1;
+  const double rpp = i + 1 < ni ? x4 : x3;
+  const double tm = i > 1 ? (x1 + beta * (rmm + x2)) / den : x2;
+  const double tp = i < ni ? (x3 + beta * (x2 + rpp)) / den : x2;
+  return (x2 + beta * (tm + tp)) / den;
+}
+
 // Direction d = 1 (j) or 2 (k) of the smoothing, marching: one thread per line and
 // slot (blockIdx.y), lanes along i, down the line.  Each residual is read once and
 // each first-sweep value T(j) formed once, by irs_line's expressions with its end
 // copies (R(0) = R(1), R(n+1) = R(n); T(0) = R(1), T(n+1) = R(n)), so the result is
-// irs_line's bit for bit.  Workgroups wl[b] .. wl[b+1]-1 take block b's lines.
-template <int NC>
+// irs_line's bit for bit.  FOLD_I: src holds the residual before the smoothing
+// along i, and each value the march reads is that smoothing's at the cell
+// (irs_i_value), formed from the line's five neighbours along i, so the
+// direction-i pass is never written and read back.  Workgroups wl[b] .. wl[b+1]-1
+// take block b's lines.
+template <int NC, bool FOLD_I = false>
 __global__ void __launch_bounds__(kThreads) k_irs_march(FamArgs a, int d, const double* __restrict__ src,
                                                         double* __restrict__ dst, const int* __restrict__ wl, int nb) {
   const int w = (int)blockIdx.x;
@@ -828,8 +844,10 @@ __global__ void __launch_bounds__(kThreads) k_irs_march(FamArgs a, int d, const 
   double* __restrict__ D = dst + (long long)v * a.N;
   const double beta = a.irs_beta;
   const double den = 1.0 + 2.0 * beta;
-  double r0 = R[g];                           // R(m)
-  double rp = n > 1 ? R[g + s] : r0;          // R(m+1)
+  const int ni = B.ni;
+  auto rd = [&](long long gg) { return FOLD_I ? irs_i_value(R, gg, i, ni, beta, den) : R[gg]; };
+  double r0 = rd(g);                          // R(m)
+  double rp = n > 1 ? rd(g + s) : r0;         // R(m+1)
   double tm = r0;                             // T(m-1)
   double tc = (r0 + beta * (r0 + rp)) / den;  // T(m)
   int m = 1;
@@ -838,7 +856,7 @@ __global__ void __launch_bounds__(kThreads) k_irs_march(FamArgs a, int d, const 
   for (; m + 9 <= n; m += 8) {
     double x[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) x[q] = R[g + (2 + q) * s];
+    for (int q = 0; q < 8; ++q) x[q] = rd(g + (2 + q) * s);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const double tp = (rp + beta * (r0 + x[q])) / den;
@@ -848,7 +866,7 @@ __global__ void __launch_bounds__(kThreads) k_irs_march(FamArgs a, int d, const 
   }
   // the rest of the interior: R(m+2) is a cell of the line
   for (; m + 2 <= n; ++m) {
-    const double rpp = R[g + 2 * s];
+    const double rpp = rd(g + 2 * s);
     const double tp = (rp + beta * (r0 + rpp)) / den;
     D[g] = (r0 + beta * (tm + tp)) / den;
     tm = tc; tc = tp; r0 = rp; rp = rpp; g += s;
@@ -860,6 +878,73 @@ __global__ void __launch_bounds__(kThreads) k_irs_march(FamArgs a, int d, const 
   }
   // m = n: T(n+1) = R(n)
   D[g] = (r0 + beta * (tm + r0)) / den;
+}
+
+// The last direction of the smoothing (d = 1 or 2) fused with the RK update: one
+// thread per line (lanes along i) and every slot, down the line.  Each cell's
+// smoothed residual is formed by k_irs_march's expressions, bit for bit, and goes
+// straight into update_cell, so it is never written and read back.  FOLD_I (d =
+// 1): src holds the residual before the smoothing along i, formed here as in
+// k_irs_march.  Workgroups wl[b] .. wl[b+1]-1 take block b's lines, as in
+// k_irs_march.
+template <int M, bool FOLD_I = false>
+__global__ void __launch_bounds__(kThreads, 2) k_irs_march_update(FamArgs a, int d, const double* __restrict__ src,
+                                                                  const int* __restrict__ wl, int nb) {
+  constexpr int NC = Closure<M>::NC;
+  const int w = (int)blockIdx.x;
+  const int b = block_of_workgroup(wl, nb, w);
+  const BlkDev& B = a.blk[b];
+  const int ni = B.ni, no = d == 1 ? B.nk : B.nj;
+  const long long t = (long long)(w - wl[b]) * blockDim.x + threadIdx.x;
+  if (t >= (long long)ni * no) return;
+  const int i = (int)(t % ni) + 1, o = (int)(t / ni) + 1;
+  const int n = d == 1 ? B.nj : B.nk;
+  const long long s = dir_stride(B, d);
+  long long g = d == 1 ? gidx(B, i, 1, o) : gidx(B, i, o, 1);
+  const long long N = a.N;
+  const double beta = a.irs_beta;
+  const double den = 1.0 + 2.0 * beta;
+  auto rd = [&](int v, long long gg) {
+    return FOLD_I ? irs_i_value(src + (long long)v * N, gg, i, ni, beta, den) : src[(long long)v * N + gg];
+  };
+  // per slot: R(m), R(m+1), T(m-1), T(m) of the cell m being finished
+  double r0[NC], rp[NC], tm[NC], tc[NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    r0[v] = rd(v, g);
+    rp[v] = n > 1 ? rd(v, g + s) : r0[v];
+    tm[v] = r0[v];
+    tc[v] = (r0[v] + beta * (r0[v] + rp[v])) / den;
+  }
+  for (int m = 1; m <= n; ++m) {
+    double out[NC];
+    if (m + 2 <= n) {
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        const double rpp = rd(v, g + 2 * s);
+        const double tp = (rp[v] + beta * (r0[v] + rpp)) / den;
+        out[v] = (r0[v] + beta * (tm[v] + tp)) / den;
+        tm[v] = tc[v];
+        tc[v] = tp;
+        r0[v] = rp[v];
+        rp[v] = rpp;
+      }
+    } else if (m + 1 == n) {  // R(n+1) = R(n)
+#pragma unroll
+      for (int v = 0; v < NC; ++v) {
+        const double tp = (rp[v] + beta * (r0[v] + rp[v])) / den;
+        out[v] = (r0[v] + beta * (tm[v] + tp)) / den;
+        tm[v] = tc[v];
+        r0[v] = rp[v];
+      }
+    } else {  // m = n: T(n+1) = R(n)
+#pragma unroll
+      for (int v = 0; v < NC; ++v) out[v] = (r0[v] + beta * (tm[v] + r0[v])) / den;
+    }
+    const int j = d == 1 ? m : o, k = d == 1 ? o : m;
+    update_cell<M>(a, B, i, j, k, g, out);
+    g += s;
+  }
 }
 
 // the dt pass of every block of the family in one launch (workgroups wg[b] ..
