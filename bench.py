@@ -56,19 +56,22 @@ def stage_kernel_bytes(case, batched: bool) -> float:
 
 
 def path_bytes(case) -> float:
-    """Compulsory HBM bytes of the whole step per family-cell-stage, fused: each
+    """HBM bytes the whole step must move per family-cell-stage on this path: each
     family's U_stage, U_0 and U_next (3 nc), the geometry (3-D 16 doubles), the gas
-    (9) when coupled, with IRS the residual materialised and read and written once
-    per direction ((2 ndir + 2) nc), the per-family dt read each stage (steady),
-    and the once-per-step dt pass spread over the stages."""
+    (9) when coupled, the per-family dt read each stage (steady), the
+    once-per-step dt pass spread over the stages, and with IRS the residual
+    buffers written and read once each (the stage's residual, and one more with
+    three resolved directions or with one: 2 nc per buffer) plus U_stage read
+    again by the update (nc)."""
     cfg = case.cfg
     nd = cfg.ndir
     g = {1: 6, 2: 11, 3: 16}[nd] + (9 if cfg.coupled else 0)
     b = g
     dtp = 0
+    nbuf = 1 if nd == 2 else 2
     for f in case.fams:
         nc = native.NCOND[f.closure]
-        b += 3 * nc + ((2 * nd + 2) * nc if cfg.irs else 0) + (0 if cfg.time_accurate else 1)
+        b += 3 * nc + ((2 * nbuf + 1) * nc if cfg.irs else 0) + (0 if cfg.time_accurate else 1)
         dtp += nc + (1 if cfg.coupled else 0) + (0 if cfg.time_accurate else 1)
     b += (dtp + 3 * nd) / cfg.nrk
     return 8.0 * b / len(case.fams)

@@ -27,13 +27,47 @@ template <> struct StageWaves<2> { static constexpr int value = ICE_AG_WAVES; };
 constexpr int kMlpWaveLds = 64 * 16 + 64 * 64 + 64 * 2 * 2;  // in uint16_t
 constexpr int kMlpLds = kMlpWaveLds * (kThreads / 64);
 
+// The network's forward weights (BF16: W1t, W2t, W3t) and biases (FP32: b1, b2,
+// b3) as a workgroup keeps them in LDS
+constexpr int kMlpWts = kMlpH * kMlpK1 + kMlpH * kMlpH + kMlpN3 * kMlpH;  // 6144
+constexpr int kMlpBias = kMlpH + kMlpH + kMlpN3;                            // 144
+
+// Copies the network's weights and biases into the workgroup's LDS (when load;
+// every thread of the workgroup calls this, and it ends with a barrier) and
+// returns the network pointing at that copy, so that every product of the
+// forward pass reads LDS instead of memory.  The copy holds the same values, so
+// the outputs are the same bits.
+ICE_DEV MlpDev mlp_in_lds(const MlpDev& w, bool load, uint16_t* wts, float* bias) {
+  constexpr int o2 = kMlpH * kMlpK1, o3 = o2 + kMlpH * kMlpH;
+  if (load) {
+    const uint32_t* s1 = (const uint32_t*)w.w1t;
+    const uint32_t* s2 = (const uint32_t*)w.w2t;
+    const uint32_t* s3 = (const uint32_t*)w.w3t;
+    uint32_t* d = (uint32_t*)wts;
+    for (int e = threadIdx.x; e < kMlpWts / 2; e += blockDim.x)
+      d[e] = e < o2 / 2 ? s1[e] : (e < o3 / 2 ? s2[e - o2 / 2] : s3[e - o3 / 2]);
+    for (int e = threadIdx.x; e < kMlpBias; e += blockDim.x)
+      bias[e] = e < kMlpH ? w.b1[e] : (e < 2 * kMlpH ? w.b2[e - kMlpH] : w.b3[e - 2 * kMlpH]);
+    __syncthreads();
+  }
+  MlpDev m = w;
+  m.w1t = wts;
+  m.w2t = wts + o2;
+  m.w3t = wts + o3;
+  m.b1 = bias;
+  m.b2 = bias + kMlpH;
+  m.b3 = bias + 2 * kMlpH;
+  return m;
+}
+
 // Seed of the interface solve of the calling thread's cell.  mat.surrogate:
 // 0 the plain solve (0), 2 the equilibrium rate (1), 1 the network's rate ratio
 // for the cell, clamped to [1e-2, 1].  With 1, every thread of the workgroup
 // calls this (the wave's 64 lanes feed one MLP pass on the MFMA units, skipped
-// when no lane's cell needs the solve).
+// when no lane's cell needs the solve).  net: the network's weights (a.mlp, or
+// its copy in LDS, mlp_in_lds).
 template <int M, bool TAB = true>
-ICE_DEV float stage_seed(const FamArgs& a, bool has_cell, long long g, uint16_t* lds) {
+ICE_DEV float stage_seed(const FamArgs& a, bool has_cell, long long g, uint16_t* lds, const MlpDev& net) {
   constexpr int NC = Closure<M>::NC;
   if (a.mat.surrogate == 2) return 1.0f;
   if (a.mat.surrogate != 1 || !a.mlp_on || a.mat.evap == 0 || a.mat.intf != 1) return 0.0f;
@@ -63,7 +97,7 @@ ICE_DEV float stage_seed(const FamArgs& a, bool has_cell, long long g, uint16_t*
   for (int k = 0; k < 16; ++k) x[l * 16 + k] = k < kMlpIn ? f2bf(f[k]) : (uint16_t)0;
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
-  mlp_forward_wave(a.mlp, x, h, h, out);
+  mlp_forward_wave(net, x, h, h, out);
   float r = out[l * kMlpOut];
   r = r < 1e-2f ? 1e-2f : (r > 1.0f ? 1.0f : r);
   return need ? r : 0.0f;
@@ -650,12 +684,17 @@ template <int M, bool WRITE_R>
 __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_lk_finish(FamArgs a) {
   constexpr int NC = Closure<M>::NC;
   __shared__ __attribute__((aligned(16))) uint16_t mlds[kMlpLds];
+  __shared__ __attribute__((aligned(16))) uint16_t mwts[kMlpWts];
+  __shared__ float mbias[kMlpBias];
+  // the workgroup's waves run many network passes: its weights are read from LDS
+  const MlpDev net =
+      mlp_in_lds(a.mlp, a.mat.surrogate == 1 && a.mlp_on && a.mat.evap != 0 && a.mat.intf == 1, mwts, mbias);
   const unsigned n0 = *a.lkq_n, n = n0 < (unsigned)a.lkq_cap ? n0 : (unsigned)a.lkq_cap;
   for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const unsigned idx = base + threadIdx.x;
     const bool has = idx < n;
     const long long g = has ? a.lkq_g[idx] : 0ll;
-    const float seed = stage_seed<M, false>(a, has, g, mlds);
+    const float seed = stage_seed<M, false>(a, has, g, mlds, net);
     if (has) {
       double r[NC];
 #pragma unroll
@@ -684,7 +723,7 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_stage(FamArg
   float seed = 0.0f;
   if constexpr (CP == 2) {
     __shared__ __attribute__((aligned(16))) uint16_t mlds[kMlpLds];
-    seed = stage_seed<M>(a, t < Bv.cells, g, mlds);
+    seed = stage_seed<M>(a, t < Bv.cells, g, mlds, a.mlp);
   }
   if (t >= Bv.cells) return;
   double r[NC];
