@@ -608,12 +608,14 @@ ICE_DEV void update_cell(const FamArgs& a, const BlkDev& B, int i, int j, int k,
   }
 }
 
-// The end of one cell's stage from its flux sum r: the source (coupled, seeded by
-// seed), the residual (Lib_Residual.f90:147-156), the imports (add_births) and the
-// frozen slots, then the RK update, or the residual for smoothing (WRITE_R).
-template <int M, bool TAB, int CP, bool WRITE_R, bool LK = true>
-ICE_DEV void finish_cell(const FamArgs& a, const BlkDev& Bref, int ci, int cj, int ck, long long g, double* r,
-                         float seed) {
+// The end of one cell's stage: the source (coupled, seeded by seed), then the
+// residual from the flux sum, which r_in(r) hands over only after the source so
+// that it is not held through the source (Lib_Residual.f90:147-156), the imports
+// (add_births) and the frozen slots, then the RK update, or the residual for
+// smoothing (WRITE_R).
+template <int M, bool TAB, int CP, bool WRITE_R, bool LK, class RIn>
+ICE_DEV void finish_cell_with(const FamArgs& a, const BlkDev& Bref, int ci, int cj, int ck, long long g, RIn r_in,
+                              float seed) {
   constexpr int NC = Closure<M>::NC;
   double src[NC];
   if constexpr (CP != 0) {
@@ -626,6 +628,8 @@ ICE_DEV void finish_cell(const FamArgs& a, const BlkDev& Bref, int ci, int cj, i
 #pragma unroll
     for (int v = 0; v < NC; ++v) src[v] = 0.0;
   }
+  double r[NC];
+  r_in(r);
   double vol = a.vol[g];
   double dt = a.time_accurate ? a.st->dtglobal : a.dt[g];
 #pragma unroll
@@ -637,6 +641,20 @@ ICE_DEV void finish_cell(const FamArgs& a, const BlkDev& Bref, int ci, int cj, i
   } else {
     update_cell<M, TAB>(a, Bref, ci, cj, ck, g, r);
   }
+}
+
+// finish_cell_with the flux sum r already in hand
+template <int M, bool TAB, int CP, bool WRITE_R, bool LK = true>
+ICE_DEV void finish_cell(const FamArgs& a, const BlkDev& Bref, int ci, int cj, int ck, long long g, const double* r,
+                         float seed) {
+  constexpr int NC = Closure<M>::NC;
+  finish_cell_with<M, TAB, CP, WRITE_R, LK>(
+      a, Bref, ci, cj, ck, g,
+      [&](double* o) {
+#pragma unroll
+        for (int v = 0; v < NC; ++v) o[v] = r[v];
+      },
+      seed);
 }
 
 // Whether the cell's source runs the Langmuir-Knudsen interface solve (lk_needed)
@@ -696,13 +714,17 @@ __global__ void __launch_bounds__(kThreads, StageWaves<M>::value) k_lk_finish(Fa
     const long long g = has ? a.lkq_g[idx] : 0ll;
     const float seed = stage_seed<M, false>(a, has, g, mlds, net);
     if (has) {
-      double r[NC];
-#pragma unroll
-      for (int v = 0; v < NC; ++v) r[v] = a.lkq_r[(long long)v * a.lkq_cap + idx];
       const BlkDev& B = a.blk[a.lkq_b[idx]];
       const long long loc = g - B.off;
       const int i = (int)(loc % B.NI) - 1, j = (int)((loc / B.NI) % B.NJ) - 1, k = (int)(loc / ((long long)B.NI * B.NJ)) - 1;
-      finish_cell<M, false, 2, WRITE_R>(a, B, i, j, k, g, r, seed);
+      // the queued flux sum is read after the solve, which then has its registers
+      finish_cell_with<M, false, 2, WRITE_R, true>(
+          a, B, i, j, k, g,
+          [&](double* o) {
+#pragma unroll
+            for (int v = 0; v < NC; ++v) o[v] = a.lkq_r[(long long)v * a.lkq_cap + idx];
+          },
+          seed);
     }
   }
 }
@@ -807,6 +829,7 @@ ICE_DEV void irs_line(const FamArgs& a, const double* __restrict__ src, long lon
                       double* out) {
   const double beta = a.irs_beta;
   const double den = 1.0 + 2.0 * beta;
+  const SharedDiv byd = div_prepare(den);  // every quotient of the smoothing divides by den
   double x[NC][5];
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
@@ -819,9 +842,9 @@ ICE_DEV void irs_line(const FamArgs& a, const double* __restrict__ src, long lon
     const double r0 = x[v][2], rm = x[v][1], rp = x[v][3];
     const double rmm = m - 1 > 1 ? x[v][0] : rm;
     const double rpp = m + 1 < n ? x[v][4] : rp;
-    const double tm = m > 1 ? (rm + beta * (rmm + r0)) / den : r0;
-    const double tp = m < n ? (rp + beta * (r0 + rpp)) / den : r0;
-    out[v] = (r0 + beta * (tm + tp)) / den;
+    const double tm = m > 1 ? div_shared(rm + beta * (rmm + r0), byd) : r0;
+    const double tp = m < n ? div_shared(rp + beta * (r0 + rpp), byd) : r0;
+    out[v] = div_shared(r0 + beta * (tm + tp), byd);
   }
 }
 
@@ -847,13 +870,14 @@ __global__ void __launch_bounds__(kThreads) k_irs(FamArgs a, int d, const double
 // The value the smoothing along i leaves at cell g (index i of ni along i) for one
 // slot Rv: irs_line's expressions for that slot, bit for bit.  g-2 .. g+2 are all
 // inside the block's padded box, so the loads are unconditional and the ends select.
-ICE_DEV double irs_i_value(const double* __restrict__ Rv, long long g, int i, int ni, double beta, double den) {
+ICE_DEV double irs_i_value(const double* __restrict__ Rv, long long g, int i, int ni, double beta,
+                            const SharedDiv& byd) {
   const double x0 = Rv[g - 2], x1 = Rv[g - 1], x2 = Rv[g], x3 = Rv[g + 1], x4 = Rv[g + 2];
   const double rmm = i - 1 > 1 ? x0 : x1;
   const double rpp = i + 1 < ni ? x4 : x3;
-  const double tm = i > 1 ? (x1 + beta * (rmm + x2)) / den : x2;
-  const double tp = i < ni ? (x3 + beta * (x2 + rpp)) / den : x2;
-  return (x2 + beta * (tm + tp)) / den;
+  const double tm = i > 1 ? div_shared(x1 + beta * (rmm + x2), byd) : x2;
+  const double tp = i < ni ? div_shared(x3 + beta * (x2 + rpp), byd) : x2;
+  return div_shared(x2 + beta * (tm + tp), byd);
 }
 
 // Direction d = 1 (j) or 2 (k) of the smoothing, marching: one thread per line and
@@ -883,12 +907,13 @@ __global__ void __launch_bounds__(kThreads) k_irs_march(FamArgs a, int d, const 
   double* __restrict__ D = dst + (long long)v * a.N;
   const double beta = a.irs_beta;
   const double den = 1.0 + 2.0 * beta;
+  const SharedDiv byd = div_prepare(den);  // every quotient of the smoothing divides by den
   const int ni = B.ni;
-  auto rd = [&](long long gg) { return FOLD_I ? irs_i_value(R, gg, i, ni, beta, den) : R[gg]; };
+  auto rd = [&](long long gg) { return FOLD_I ? irs_i_value(R, gg, i, ni, beta, byd) : R[gg]; };
   double r0 = rd(g);                          // R(m)
   double rp = n > 1 ? rd(g + s) : r0;         // R(m+1)
   double tm = r0;                             // T(m-1)
-  double tc = (r0 + beta * (r0 + rp)) / den;  // T(m)
+  double tc = div_shared(r0 + beta * (r0 + rp), byd);  // T(m)
   int m = 1;
   // interior, eight cells at a time with their loads issued together: R(m+2) ..
   // R(m+9) are cells of the line
@@ -898,25 +923,25 @@ __global__ void __launch_bounds__(kThreads) k_irs_march(FamArgs a, int d, const 
     for (int q = 0; q < 8; ++q) x[q] = rd(g + (2 + q) * s);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const double tp = (rp + beta * (r0 + x[q])) / den;
-      D[g] = (r0 + beta * (tm + tp)) / den;
+      const double tp = div_shared(rp + beta * (r0 + x[q]), byd);
+      D[g] = div_shared(r0 + beta * (tm + tp), byd);
       tm = tc; tc = tp; r0 = rp; rp = x[q]; g += s;
     }
   }
   // the rest of the interior: R(m+2) is a cell of the line
   for (; m + 2 <= n; ++m) {
     const double rpp = rd(g + 2 * s);
-    const double tp = (rp + beta * (r0 + rpp)) / den;
-    D[g] = (r0 + beta * (tm + tp)) / den;
+    const double tp = div_shared(rp + beta * (r0 + rpp), byd);
+    D[g] = div_shared(r0 + beta * (tm + tp), byd);
     tm = tc; tc = tp; r0 = rp; rp = rpp; g += s;
   }
   if (m + 1 == n) {  // m = n-1: R(n+1) = R(n)
-    const double tp = (rp + beta * (r0 + rp)) / den;
-    D[g] = (r0 + beta * (tm + tp)) / den;
+    const double tp = div_shared(rp + beta * (r0 + rp), byd);
+    D[g] = div_shared(r0 + beta * (tm + tp), byd);
     tm = tc; r0 = rp; g += s; ++m;
   }
   // m = n: T(n+1) = R(n)
-  D[g] = (r0 + beta * (tm + r0)) / den;
+  D[g] = div_shared(r0 + beta * (tm + r0), byd);
 }
 
 // The last direction of the smoothing (d = 1 or 2) fused with the RK update: one
@@ -943,8 +968,12 @@ __global__ void __launch_bounds__(kThreads, 2) k_irs_march_update(FamArgs a, int
   const long long N = a.N;
   const double beta = a.irs_beta;
   const double den = 1.0 + 2.0 * beta;
+  // irs_i_value divides through byd; the march's own quotients are plain
+  // divisions (the shared-divisor form, inlined for every slot of every case,
+  // measured slower in this kernel)
+  const SharedDiv byd = div_prepare(den);
   auto rd = [&](int v, long long gg) {
-    return FOLD_I ? irs_i_value(src + (long long)v * N, gg, i, ni, beta, den) : src[(long long)v * N + gg];
+    return FOLD_I ? irs_i_value(src + (long long)v * N, gg, i, ni, beta, byd) : src[(long long)v * N + gg];
   };
   // per slot: R(m), R(m+1), T(m-1), T(m) of the cell m being finished
   double r0[NC], rp[NC], tm[NC], tc[NC];
